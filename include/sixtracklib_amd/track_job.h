@@ -1,0 +1,159 @@
+/* sixtracklib_amd/track_job.h -- the track-job C API (drop-in boundary).
+ *
+ * Replaces the reference's legacy track-job API used by the Python
+ * ``sixtracklib.TrackJob`` (sixtracklib/common/track_job.h:139-470,
+ * implemented in common/internal/track_job.cpp; ctypes prototypes in
+ * python/sixtracklib/stcommon.py:1334-1375).  Names, argument meaning,
+ * return conventions and ownership rules are the reference's:
+ *   - constructors return NULL on failure;
+ *   - track functions return NS(track_status_t): 0 success, -1 failure,
+ *     per-particle statuses OR'ed; a NULL job gives -1; particle loss is
+ *     not an error;
+ *   - the caller owns the particle and beam-element buffers, the job keeps
+ *     pointers to them; a NULL output buffer makes the job allocate and own
+ *     one when output is needed (element-by-element or BeamMonitors);
+ *   - collect writes device results back into the caller's buffers in
+ *     place; push uploads host changes.
+ * The architecture string is "hip" (also accepted: "amd", "amdgpu",
+ * "gpu").  There is no CPU architecture in this library: "cpu" and
+ * "opencl" are rejected (constructor returns NULL).
+ * config_str tokens (space or comma separated): "fma=0|1" (default 0:
+ * results bit-identical to the reference CPU tracker; 1: fused
+ * multiply-add build, within 1e-13 relative), "block=64..256",
+ * "device=N", "ebe_rolling=0|1" (default 1, as the reference job).
+ */
+#ifndef SIXTRACKLIB_AMD_TRACK_JOB_H__
+#define SIXTRACKLIB_AMD_TRACK_JOB_H__
+
+#include "sixtracklib_amd/definitions.h"
+#include "sixtracklib_amd/buffer.h"
+#include "sixtracklib_amd/particles.h"
+
+#if defined( __cplusplus )
+extern "C" {
+#endif
+
+typedef struct NS(TrackJobBase) NS(TrackJobBase);
+typedef NS(TrackJobBase) NS(TrackJob);
+
+/* track_job.h:139 */
+ST_API NS(TrackJobBase)* NS(TrackJob_create)( char const* arch, char const* config_str );
+/* track_job.h:143 */
+ST_API NS(TrackJobBase)* NS(TrackJob_new)( char const* arch,
+    NS(Buffer)* particles_buffer, NS(Buffer)* beam_elements_buffer,
+    char const* config_str );
+/* track_job.h:149 */
+ST_API NS(TrackJobBase)* NS(TrackJob_new_with_output)( char const* arch,
+    NS(Buffer)* particles_buffer, NS(Buffer)* beam_elements_buffer,
+    NS(Buffer)* output_buffer, NS(buffer_size_t) until_turn_elem_by_elem,
+    char const* config_str );
+/* track_job.h:157 */
+ST_API NS(TrackJobBase)* NS(TrackJob_new_detailed)( char const* arch,
+    NS(Buffer)* particles_buffer, NS(buffer_size_t) num_particle_sets,
+    NS(buffer_size_t) const* particle_set_indices_begin,
+    NS(Buffer)* beam_elements_buffer, NS(Buffer)* output_buffer,
+    NS(buffer_size_t) until_turn_elem_by_elem, char const* config_str );
+/* track_job.h:169 */
+ST_API void NS(TrackJob_delete)( NS(TrackJobBase)* job );
+
+/* track_job.h:172-184 */
+ST_API NS(track_status_t) NS(TrackJob_track_until)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) until_turn );
+ST_API NS(track_status_t) NS(TrackJob_track_elem_by_elem)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) until_turn );
+ST_API NS(track_status_t) NS(TrackJob_track_line)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) begin_idx, NS(buffer_size_t) end_idx, bool finish_turn );
+
+/* track_job.h:188-244 */
+ST_API void NS(TrackJob_collect)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_collect_detailed)( NS(TrackJobBase)* job, NS(track_job_collect_flag_t) flags );
+ST_API void NS(TrackJob_collect_particles)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_collect_beam_elements)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_collect_output)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_enable_collect_particles)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_disable_collect_particles)( NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_is_collecting_particles)( const NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_enable_collect_beam_elements)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_disable_collect_beam_elements)( NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_is_collecting_beam_elements)( const NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_enable_collect_output)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_disable_collect_output)( NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_is_collecting_output)( const NS(TrackJobBase)* job );
+ST_API NS(track_job_collect_flag_t) NS(TrackJob_get_collect_flags)( const NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_set_collect_flags)( NS(TrackJobBase)* job, NS(track_job_collect_flag_t) flags );
+ST_API bool NS(TrackJob_requires_collecting)( const NS(TrackJobBase)* job );
+
+/* track_job.h:248-262 */
+ST_API void NS(TrackJob_push)( NS(TrackJobBase)* job, NS(track_job_push_flag_t) flags );
+ST_API void NS(TrackJob_push_particles)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_push_beam_elements)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_push_output)( NS(TrackJobBase)* job );
+
+/* track_job.h:300-336 */
+ST_API void NS(TrackJob_clear)( NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_reset)( NS(TrackJobBase)* job, NS(Buffer)* particles_buffer,
+    NS(Buffer)* beam_elements_buffer, NS(Buffer)* output_buffer );
+ST_API bool NS(TrackJob_reset_with_output)( NS(TrackJobBase)* job,
+    NS(Buffer)* particles_buffer, NS(Buffer)* beam_elements_buffer,
+    NS(Buffer)* output_buffer, NS(buffer_size_t) until_turn_elem_by_elem );
+ST_API bool NS(TrackJob_reset_detailed)( NS(TrackJobBase)* job,
+    NS(Buffer)* particles_buffer, NS(buffer_size_t) num_particle_sets,
+    NS(buffer_size_t) const* particle_set_indices_begin,
+    NS(Buffer)* beam_elements_buffer, NS(Buffer)* output_buffer,
+    NS(buffer_size_t) until_turn_elem_by_elem );
+ST_API bool NS(TrackJob_select_particle_set)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) particle_set_index );
+ST_API bool NS(TrackJob_assign_output_buffer)( NS(TrackJobBase)* job,
+    NS(Buffer)* output_buffer );
+
+/* track_job.h:342-470 introspection */
+ST_API NS(track_job_type_t) NS(TrackJob_get_type_id)( const NS(TrackJobBase)* job );
+ST_API char const* NS(TrackJob_get_type_str)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_has_device_id_str)( const NS(TrackJobBase)* job );
+ST_API char const* NS(TrackJob_get_device_id_str)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_has_config_str)( const NS(TrackJobBase)* job );
+ST_API char const* NS(TrackJob_get_config_str)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_num_particle_sets)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) const* NS(TrackJob_get_particle_set_indices_begin)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) const* NS(TrackJob_get_particle_set_indices_end)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_particle_set_index)( const NS(TrackJobBase)* job, NS(buffer_size_t) n );
+ST_API NS(buffer_size_t) NS(TrackJob_get_total_num_particles_in_sets)( const NS(TrackJobBase)* job );
+ST_API NS(particle_index_t) NS(TrackJob_get_min_particle_id)( const NS(TrackJobBase)* job );
+ST_API NS(particle_index_t) NS(TrackJob_get_max_particle_id)( const NS(TrackJobBase)* job );
+ST_API NS(particle_index_t) NS(TrackJob_get_min_element_id)( const NS(TrackJobBase)* job );
+ST_API NS(particle_index_t) NS(TrackJob_get_max_element_id)( const NS(TrackJobBase)* job );
+ST_API NS(particle_index_t) NS(TrackJob_get_min_initial_turn_id)( const NS(TrackJobBase)* job );
+ST_API NS(particle_index_t) NS(TrackJob_get_max_initial_turn_id)( const NS(TrackJobBase)* job );
+ST_API NS(Buffer)* NS(TrackJob_get_particles_buffer)( NS(TrackJobBase)* job );
+ST_API NS(Buffer)* NS(TrackJob_get_beam_elements_buffer)( NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_has_output_buffer)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_owns_output_buffer)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_has_elem_by_elem_output)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_has_beam_monitor_output)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_beam_monitor_output_buffer_offset)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_elem_by_elem_output_buffer_offset)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_num_elem_by_elem_turns)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_num_beam_monitors)( const NS(TrackJobBase)* job );
+ST_API NS(Buffer)* NS(TrackJob_get_output_buffer)( NS(TrackJobBase)* job );
+ST_API NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* job );
+
+/* Output-buffer sizing used by the Python binding before it builds its
+ * own output buffer (output/output_buffer.c:44-131 + 288-363):
+ * reports the particle-set sizes the job will need, in the order the
+ * output buffer must hold them (element-by-element set first, then one
+ * per BeamMonitor).  Returns the number of sets, -1 on error. */
+ST_API int64_t NS(TrackJob_required_output_sets)( NS(Buffer)* particles_buffer,
+    NS(buffer_size_t) num_particle_sets,
+    NS(buffer_size_t) const* particle_set_indices_begin,
+    NS(Buffer)* beam_elements_buffer, NS(buffer_size_t) until_turn_elem_by_elem,
+    int64_t* out_sizes, int64_t max_sets, int64_t* has_elem_by_elem );
+
+/* Device/runtime introspection (no reference counterpart). */
+ST_API int NS(Hip_num_devices)( void );
+ST_API char const* NS(Hip_last_error_str)( void );
+
+#if defined( __cplusplus )
+}
+#endif
+
+#endif /* SIXTRACKLIB_AMD_TRACK_JOB_H__ */

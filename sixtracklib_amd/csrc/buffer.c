@@ -1,0 +1,458 @@
+/* CObjects managed buffer, host side (C99).
+ *
+ * Format: sixtracklib/common/buffer/managed_buffer_minimal.h:396-466,
+ * 629-865 (8-word header, sections {max_size, num_entities}), objects
+ * buffer/buffer_object.h:24-30, relocation managed_buffer_remap.h:139-178,
+ * 345-550, 644-681.  The implementation is this project's own: one
+ * contiguous allocation laid out as header | slots | objects | dataptrs |
+ * garbage, grown by re-layout + relocation.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "sixtracklib_amd/buffer.h"
+
+#define HDR_BYTES 64u
+
+struct NS(Buffer)
+{
+    unsigned char* data;      /* 8-byte aligned start of the image     */
+    uint64_t       capacity;  /* bytes available at data               */
+    void*          alloc;     /* allocation to free (owned buffers)    */
+    int            owns;
+};
+
+/* ----------------------------------------------------------------------- */
+
+static uint64_t* hdr_of( unsigned char* d ) { return ( uint64_t* )d; }
+
+/* word pointer to section sid (3..6), located via offsets from the stored
+ * base so it works on images that are not relocated to their address. */
+static uint64_t* sec_of( unsigned char* d, int sid )
+{
+    uint64_t const* h = ( uint64_t const* )d;
+    return ( uint64_t* )( d + ( h[ sid ] - h[ 0 ] ) );
+}
+
+static void layout_init( unsigned char* d, uint64_t n_slots, uint64_t n_objs,
+                         uint64_t n_ptrs, uint64_t n_garb )
+{
+    uint64_t* h = hdr_of( d );
+    uint64_t const base = ( uint64_t )( uintptr_t )d;
+    uint64_t const sizes[ 4 ] = { 16u + 8u * n_slots, 16u + 24u * n_objs,
+                                  16u + 8u * n_ptrs, 16u + 16u * n_garb };
+    uint64_t off = HDR_BYTES;
+    int k;
+    memset( d, 0, HDR_BYTES );
+    h[ 0 ] = base;
+    h[ 2 ] = HDR_BYTES;
+    for( k = 0 ; k < 4 ; ++k )
+    {
+        uint64_t* s = ( uint64_t* )( d + off );
+        h[ 3 + k ] = base + off;
+        s[ 0 ] = sizes[ k ];
+        s[ 1 ] = 0u;
+        off += sizes[ k ];
+    }
+    h[ 1 ] = off;
+}
+
+static uint64_t layout_bytes( uint64_t n_slots, uint64_t n_objs,
+                              uint64_t n_ptrs, uint64_t n_garb )
+{
+    return HDR_BYTES + 16u + 8u * n_slots + 16u + 24u * n_objs +
+           16u + 8u * n_ptrs + 16u + 16u * n_garb;
+}
+
+static uint64_t sec_capacity( unsigned char* d, int sid, uint64_t ent )
+{
+    return ( sec_of( d, sid )[ 0 ] - 16u ) / ent;
+}
+
+int NS(ManagedBuffer_remap_to)( unsigned char* d, NS(buffer_addr_t) target )
+{
+    uint64_t* h;
+    uint64_t stored, off, n, i;
+    uint64_t *objs, *dps;
+    if( d == NULL ) return -1;
+    h = hdr_of( d );
+    if( target == 0u ) target = ( uint64_t )( uintptr_t )d;
+    if( ( target % 8u ) != 0u ) return -1;
+    stored = h[ 0 ];
+    if( stored == target ) return 0;
+    off = target - stored;                  /* modular arithmetic */
+    objs = sec_of( d, 4 );
+    dps = sec_of( d, 5 );
+    n = objs[ 1 ];
+    for( i = 0 ; i < n ; ++i ) objs[ 2 + 3 * i ] += off;
+    n = dps[ 1 ];
+    for( i = 0 ; i < n ; ++i )
+    {
+        uint64_t* slot = ( uint64_t* )( d + ( dps[ 2 + i ] - stored ) );
+        dps[ 2 + i ] += off;
+        if( *slot != 0u ) *slot += off;
+    }
+    for( i = 3 ; i <= 6 ; ++i ) h[ i ] += off;
+    h[ 0 ] = target;
+    return 0;
+}
+
+/* ----------------------------------------------------------------------- */
+
+static NS(Buffer)* buffer_alloc_owned( uint64_t bytes )
+{
+    NS(Buffer)* b = ( NS(Buffer)* )calloc( 1, sizeof( NS(Buffer) ) );
+    void* mem = NULL;
+    if( b == NULL ) return NULL;
+    if( posix_memalign( &mem, 64, bytes ) != 0 ) { free( b ); return NULL; }
+    memset( mem, 0, bytes );
+    b->alloc = mem;
+    b->data = ( unsigned char* )mem;
+    b->capacity = bytes;
+    b->owns = 1;
+    return b;
+}
+
+NS(Buffer)* NS(Buffer_new_detailed)( NS(buffer_size_t) max_objs,
+    NS(buffer_size_t) max_slots, NS(buffer_size_t) max_ptrs,
+    NS(buffer_size_t) max_garb, uint64_t flags )
+{
+    uint64_t bytes;
+    NS(Buffer)* b;
+    ( void )flags;
+    if( max_objs < 4u ) max_objs = 4u;
+    if( max_slots < 16u ) max_slots = 16u;
+    if( max_ptrs < 4u ) max_ptrs = 4u;
+    bytes = layout_bytes( max_slots, max_objs, max_ptrs, max_garb );
+    b = buffer_alloc_owned( bytes );
+    if( b != NULL ) layout_init( b->data, max_slots, max_objs, max_ptrs, max_garb );
+    return b;
+}
+
+NS(Buffer)* NS(Buffer_new)( NS(buffer_size_t) capacity )
+{
+    /* split a byte budget into sections (mostly slots) */
+    uint64_t slots = capacity / 8u;
+    uint64_t objs = slots / 16u + 4u;
+    uint64_t ptrs = slots / 16u + 4u;
+    return NS(Buffer_new_detailed)( objs, slots, ptrs, 0u, 0u );
+}
+
+NS(Buffer)* NS(Buffer_new_from_file)( char const* path )
+{
+    FILE* fp;
+    long len;
+    NS(Buffer)* b = NULL;
+    if( path == NULL ) return NULL;
+    fp = fopen( path, "rb" );
+    if( fp == NULL ) return NULL;
+    if( fseek( fp, 0, SEEK_END ) == 0 && ( len = ftell( fp ) ) >= ( long )HDR_BYTES &&
+        fseek( fp, 0, SEEK_SET ) == 0 )
+    {
+        b = buffer_alloc_owned( ( uint64_t )len );
+        if( b != NULL )
+        {
+            if( fread( b->data, 1, ( size_t )len, fp ) != ( size_t )len ||
+                hdr_of( b->data )[ 1 ] > ( uint64_t )len ||
+                NS(ManagedBuffer_remap_to)( b->data, 0u ) != 0 )
+            {
+                NS(Buffer_delete)( b );
+                b = NULL;
+            }
+        }
+    }
+    fclose( fp );
+    return b;
+}
+
+NS(Buffer)* NS(Buffer_new_on_memory)( unsigned char* data, NS(buffer_size_t) size )
+{
+    NS(Buffer)* b;
+    if( data == NULL || ( ( uintptr_t )data % 8u ) != 0u || size < HDR_BYTES )
+        return NULL;
+    if( hdr_of( data )[ 1 ] > size ) return NULL;
+    b = ( NS(Buffer)* )calloc( 1, sizeof( NS(Buffer) ) );
+    if( b == NULL ) return NULL;
+    b->data = data;
+    b->capacity = size;
+    b->owns = 0;
+    if( NS(ManagedBuffer_remap_to)( data, 0u ) != 0 ) { free( b ); return NULL; }
+    return b;
+}
+
+NS(Buffer)* NS(Buffer_new_mapped_on_cbuffer)( unsigned char* data,
+                                              NS(buffer_size_t) size )
+{
+    return NS(Buffer_new_on_memory)( data, size );
+}
+
+NS(Buffer)* NS(Buffer_new_from_copy)( const NS(Buffer)* other )
+{
+    NS(Buffer)* b;
+    uint64_t size;
+    if( other == NULL ) return NULL;
+    size = hdr_of( other->data )[ 1 ];
+    b = buffer_alloc_owned( size );
+    if( b == NULL ) return NULL;
+    memcpy( b->data, other->data, size );
+    NS(ManagedBuffer_remap_to)( b->data, 0u );
+    return b;
+}
+
+void NS(Buffer_delete)( NS(Buffer)* b )
+{
+    if( b == NULL ) return;
+    if( b->owns ) free( b->alloc );
+    free( b );
+}
+
+bool NS(Buffer_write_to_file)( const NS(Buffer)* b, char const* path )
+{
+    FILE* fp;
+    size_t n;
+    if( b == NULL || path == NULL ) return false;
+    fp = fopen( path, "wb" );
+    if( fp == NULL ) return false;
+    n = ( size_t )hdr_of( b->data )[ 1 ];
+    n = fwrite( b->data, 1, n, fp ) == n;
+    fclose( fp );
+    return n != 0;
+}
+
+bool NS(Buffer_needs_remapping)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) && ( hdr_of( b->data )[ 0 ] != ( uint64_t )( uintptr_t )b->data );
+}
+
+int NS(Buffer_remap)( NS(Buffer)* b )
+{
+    return ( b != NULL ) ? NS(ManagedBuffer_remap_to)( b->data, 0u ) : -1;
+}
+
+int NS(Buffer_clear)( NS(Buffer)* b, bool zero )
+{
+    int k;
+    if( b == NULL ) return -1;
+    for( k = 3 ; k <= 6 ; ++k )
+    {
+        uint64_t* s = sec_of( b->data, k );
+        if( zero ) memset( s + 2, 0, s[ 0 ] - 16u );
+        s[ 1 ] = 0u;
+    }
+    return 0;
+}
+
+NS(buffer_size_t) NS(Buffer_get_num_of_objects)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) ? sec_of( b->data, 4 )[ 1 ] : 0u;
+}
+
+NS(buffer_size_t) NS(Buffer_get_size)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) ? hdr_of( b->data )[ 1 ] : 0u;
+}
+
+NS(buffer_size_t) NS(Buffer_get_capacity)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) ? b->capacity : 0u;
+}
+
+NS(buffer_size_t) NS(Buffer_get_slot_size)( const NS(Buffer)* b )
+{
+    ( void )b;
+    return 8u;
+}
+
+NS(buffer_size_t) NS(Buffer_get_num_of_slots)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) ? sec_of( b->data, 3 )[ 1 ] : 0u;
+}
+
+NS(buffer_size_t) NS(Buffer_get_num_of_dataptrs)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) ? sec_of( b->data, 5 )[ 1 ] : 0u;
+}
+
+NS(buffer_addr_t) NS(Buffer_get_data_begin_addr)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) ? ( NS(buffer_addr_t) )( uintptr_t )b->data : 0u;
+}
+
+unsigned char* NS(Buffer_get_data_begin)( NS(Buffer)* b )
+{
+    return ( b != NULL ) ? b->data : NULL;
+}
+
+unsigned char const* NS(Buffer_get_const_data_begin)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) ? b->data : NULL;
+}
+
+bool NS(Buffer_owns_datastore)( const NS(Buffer)* b )
+{
+    return ( b != NULL ) && b->owns;
+}
+
+NS(Object)* NS(Buffer_get_objects_begin)( NS(Buffer)* b )
+{
+    return ( b != NULL ) ? ( NS(Object)* )( sec_of( b->data, 4 ) + 2 ) : NULL;
+}
+
+NS(Object)* NS(Buffer_get_objects_end)( NS(Buffer)* b )
+{
+    return ( b != NULL )
+        ? NS(Buffer_get_objects_begin)( b ) + NS(Buffer_get_num_of_objects)( b )
+        : NULL;
+}
+
+NS(Object)* NS(Buffer_get_object)( NS(Buffer)* b, NS(buffer_size_t) index )
+{
+    if( b == NULL || index >= NS(Buffer_get_num_of_objects)( b ) ) return NULL;
+    return NS(Buffer_get_objects_begin)( b ) + index;
+}
+
+NS(Object) const* NS(Buffer_get_const_object)( const NS(Buffer)* b,
+                                               NS(buffer_size_t) index )
+{
+    return NS(Buffer_get_object)( ( NS(Buffer)* )b, index );
+}
+
+NS(object_type_id_t) NS(Object_get_type_id)( const NS(Object)* o )
+{
+    return ( o != NULL ) ? o->type_id : ST_OBJECT_TYPE_INVALID;
+}
+
+NS(buffer_addr_t) NS(Object_get_begin_addr)( const NS(Object)* o )
+{
+    return ( o != NULL ) ? o->begin_addr : 0u;
+}
+
+NS(buffer_size_t) NS(Object_get_size)( const NS(Object)* o )
+{
+    return ( o != NULL ) ? o->size : 0u;
+}
+
+/* ----------------------------------------------------------------------- */
+/* growth: re-layout into a larger allocation and relocate the slot data */
+
+static int buffer_grow( NS(Buffer)* b, uint64_t need_slots, uint64_t need_objs,
+                        uint64_t need_ptrs )
+{
+    unsigned char* od = b->data;
+    uint64_t* osl = sec_of( od, 3 );
+    uint64_t* oob = sec_of( od, 4 );
+    uint64_t* odp = sec_of( od, 5 );
+    uint64_t const used_slots = osl[ 1 ], n_objs = oob[ 1 ], n_ptrs = odp[ 1 ];
+    uint64_t cs = sec_capacity( od, 3, 8u ), co = sec_capacity( od, 4, 24u );
+    uint64_t cp = sec_capacity( od, 5, 8u ), cg = sec_capacity( od, 6, 16u );
+    uint64_t bytes, i, delta;
+    void* mem = NULL;
+    unsigned char* nd;
+    uint64_t *nsl, *nob, *ndp;
+
+    if( !b->owns ) return -1;
+    if( cs < 64u ) cs = 64u;
+    if( co < 16u ) co = 16u;
+    if( cp < 16u ) cp = 16u;
+    while( cs < used_slots + need_slots ) cs *= 2u;
+    while( co < n_objs + need_objs ) co *= 2u;
+    while( cp < n_ptrs + need_ptrs ) cp *= 2u;
+    bytes = layout_bytes( cs, co, cp, cg );
+    if( posix_memalign( &mem, 64, bytes ) != 0 ) return -1;
+    memset( mem, 0, bytes );
+    nd = ( unsigned char* )mem;
+    layout_init( nd, cs, co, cp, cg );
+    nsl = sec_of( nd, 3 );
+    nob = sec_of( nd, 4 );
+    ndp = sec_of( nd, 5 );
+    memcpy( nsl + 2, osl + 2, used_slots * 8u );
+    nsl[ 1 ] = used_slots;
+    delta = ( uint64_t )( uintptr_t )( nsl + 2 ) - ( uint64_t )( uintptr_t )( osl + 2 );
+    memcpy( nob + 2, oob + 2, n_objs * 24u );
+    nob[ 1 ] = n_objs;
+    for( i = 0 ; i < n_objs ; ++i ) nob[ 2 + 3 * i ] += delta;
+    ndp[ 1 ] = n_ptrs;
+    for( i = 0 ; i < n_ptrs ; ++i )
+    {
+        uint64_t* slot;
+        ndp[ 2 + i ] = odp[ 2 + i ] + delta;
+        slot = ( uint64_t* )( uintptr_t )ndp[ 2 + i ];
+        if( *slot != 0u ) *slot += delta;
+    }
+    free( b->alloc );
+    b->alloc = mem;
+    b->data = nd;
+    b->capacity = bytes;
+    return 0;
+}
+
+int NS(Buffer_reserve)( NS(Buffer)* b, NS(buffer_size_t) max_objs,
+    NS(buffer_size_t) max_slots, NS(buffer_size_t) max_ptrs,
+    NS(buffer_size_t) max_garb )
+{
+    uint64_t ns, no, np;
+    ( void )max_garb;
+    if( b == NULL ) return -1;
+    ns = sec_of( b->data, 3 )[ 1 ];
+    no = sec_of( b->data, 4 )[ 1 ];
+    np = sec_of( b->data, 5 )[ 1 ];
+    if( max_slots <= sec_capacity( b->data, 3, 8u ) &&
+        max_objs <= sec_capacity( b->data, 4, 24u ) &&
+        max_ptrs <= sec_capacity( b->data, 5, 8u ) ) return 0;
+    return buffer_grow( b, max_slots > ns ? max_slots - ns : 0u,
+                        max_objs > no ? max_objs - no : 0u,
+                        max_ptrs > np ? max_ptrs - np : 0u );
+}
+
+NS(Object)* NS(Buffer_add_object)( NS(Buffer)* b, const void* handle,
+    NS(buffer_size_t) obj_size, NS(object_type_id_t) type_id,
+    NS(buffer_size_t) num_ptrs, NS(buffer_size_t) const* offsets,
+    NS(buffer_size_t) const* sizes, NS(buffer_size_t) const* counts )
+{
+    uint64_t struct_slots, nslots, i;
+    uint64_t *sl, *ob, *dp, *start, *cur;
+    uint64_t used, nobj, nptr;
+    if( b == NULL || handle == NULL || obj_size == 0u ) return NULL;
+    struct_slots = ( obj_size + 7u ) / 8u;
+    nslots = struct_slots;
+    for( i = 0 ; i < num_ptrs ; ++i )
+        nslots += ( sizes[ i ] * counts[ i ] + 7u ) / 8u;
+    sl = sec_of( b->data, 3 );
+    ob = sec_of( b->data, 4 );
+    dp = sec_of( b->data, 5 );
+    if( sl[ 1 ] + nslots > sec_capacity( b->data, 3, 8u ) ||
+        ob[ 1 ] + 1u > sec_capacity( b->data, 4, 24u ) ||
+        dp[ 1 ] + num_ptrs > sec_capacity( b->data, 5, 8u ) )
+    {
+        if( buffer_grow( b, nslots, 1u, num_ptrs ) != 0 ) return NULL;
+        sl = sec_of( b->data, 3 );
+        ob = sec_of( b->data, 4 );
+        dp = sec_of( b->data, 5 );
+    }
+    used = sl[ 1 ];
+    start = sl + 2 + used;
+    memset( start, 0, nslots * 8u );
+    memcpy( start, handle, obj_size );
+    cur = start + struct_slots;
+    nptr = dp[ 1 ];
+    for( i = 0 ; i < num_ptrs ; ++i )
+    {
+        uint64_t const bytes = sizes[ i ] * counts[ i ];
+        uint64_t* field = ( uint64_t* )( ( unsigned char* )start + offsets[ i ] );
+        const void* src = ( const void* )( uintptr_t )( *field );
+        if( src != NULL && bytes > 0u ) memcpy( cur, src, bytes );
+        *field = ( uint64_t )( uintptr_t )cur;
+        dp[ 2 + nptr ] = ( uint64_t )( uintptr_t )field;
+        ++nptr;
+        cur += ( bytes + 7u ) / 8u;
+    }
+    dp[ 1 ] = nptr;
+    sl[ 1 ] = used + nslots;
+    nobj = ob[ 1 ];
+    ob[ 2 + 3 * nobj ] = ( uint64_t )( uintptr_t )start;
+    ob[ 3 + 3 * nobj ] = type_id;
+    ob[ 4 + 3 * nobj ] = nslots * 8u;
+    ob[ 1 ] = nobj + 1u;
+    return ( NS(Object)* )( ob + 2 + 3 * nobj );
+}

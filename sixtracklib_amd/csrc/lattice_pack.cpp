@@ -1,0 +1,215 @@
+// Lattice packer: CObjects beam-element buffer -> packed device stream
+// (format in lattice.h).  Host code.
+//
+// Element data layouts read here are the reference structs
+// (be_*/be_*.h, cited in include/sixtracklib_amd/beam_elements.h); the
+// dispatch table mirrors track/track.h:209-585 (which type ids are tracked,
+// which are disabled, which mark the particle lost).
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "sixtracklib_amd/definitions.h"
+#include "sixtracklib_amd/buffer.h"
+#include "sixtracklib_amd/lattice_pack.h"
+#include "lattice.h"
+
+namespace
+{
+struct Obj { uint64_t begin, type, size; };
+
+inline uint64_t as_u64( double v ) { uint64_t u; std::memcpy( &u, &v, 8 ); return u; }
+
+inline const Obj* objects( const unsigned char* buf, int64_t* n )
+{
+    const uint64_t* h = reinterpret_cast< const uint64_t* >( buf );
+    const uint64_t* sec = reinterpret_cast< const uint64_t* >( buf + ( h[ 4 ] - h[ 0 ] ) );
+    *n = static_cast< int64_t >( sec[ 1 ] );
+    return reinterpret_cast< const Obj* >( sec + 2 );
+}
+
+// host pointer to an address stored inside the buffer (buffer relocated to
+// its own address, so stored addresses are host addresses)
+inline const uint64_t* at( uint64_t addr ) { return reinterpret_cast< const uint64_t* >( addr ); }
+
+struct Packer
+{
+    std::vector< uint64_t > w;
+    std::vector< uint32_t > offs;
+    std::vector< uint32_t > monitor_fields;   // word index of out_dev per monitor
+    uint64_t lo = 0, hi = 0;                  // buffer extent (bounds for pointers)
+
+    bool inside( uint64_t addr, uint64_t bytes ) const
+    { return addr >= lo && addr + bytes <= hi && ( addr % 8u ) == 0u; }
+
+    void rec( unsigned op, unsigned aux, const std::vector< uint64_t >& payload )
+    {
+        w.push_back( ST_MAKE_HDR( op, aux, payload.size() + 1u ) );
+        w.insert( w.end(), payload.begin(), payload.end() );
+    }
+
+    bool add( const Obj& o )
+    {
+        const uint64_t* e = at( o.begin );
+        if( !inside( o.begin, o.size ) ) return false;
+        switch( o.type )
+        {
+        case ST_OBJECT_TYPE_DRIFT: rec( ST_OP_DRIFT, 0, { e[ 0 ] } ); break;
+        case ST_OBJECT_TYPE_DRIFT_EXACT: rec( ST_OP_DRIFT_EXACT, 0, { e[ 0 ] } ); break;
+        case ST_OBJECT_TYPE_MULTIPOLE:
+        {
+            int64_t order; std::memcpy( &order, &e[ 0 ], 8 );
+            double length, hxl, hyl;
+            std::memcpy( &length, &e[ 1 ], 8 ); std::memcpy( &hxl, &e[ 2 ], 8 );
+            std::memcpy( &hyl, &e[ 3 ], 8 );
+            if( order < 0 || order > 0x7fff ) return false;
+            uint64_t const nb = 2u * static_cast< uint64_t >( order ) + 2u;
+            if( !inside( e[ 4 ], nb * 8u ) ) return false;
+            std::vector< uint64_t > pl( at( e[ 4 ] ), at( e[ 4 ] ) + nb );
+            // be_multipole/track.h:84: curvature terms only if hxl or hyl != 0
+            bool const curv = ( hxl > 0.0 ) || ( hyl > 0.0 ) || ( hxl < 0.0 ) || ( hyl < 0.0 );
+            if( curv ) { pl.push_back( e[ 2 ] ); pl.push_back( e[ 3 ] ); pl.push_back( e[ 1 ] ); }
+            rec( ST_OP_MULTIPOLE, static_cast< unsigned >( order ) | ( curv ? ST_MP_CURV_FLAG : 0u ), pl );
+            break;
+        }
+        case ST_OBJECT_TYPE_RF_MULTIPOLE:
+        {
+            int64_t order; std::memcpy( &order, &e[ 0 ], 8 );
+            if( order < 0 || order > 0x7fff ) return false;
+            uint64_t const nb = 2u * static_cast< uint64_t >( order ) + 2u;
+            if( !inside( e[ 4 ], nb * 8u ) || !inside( e[ 5 ], nb * 8u ) ) return false;
+            std::vector< uint64_t > pl = { e[ 1 ], e[ 2 ], e[ 3 ] };
+            pl.insert( pl.end(), at( e[ 4 ] ), at( e[ 4 ] ) + nb );
+            pl.insert( pl.end(), at( e[ 5 ] ), at( e[ 5 ] ) + nb );
+            rec( ST_OP_RF_MULTIPOLE, static_cast< unsigned >( order ), pl );
+            break;
+        }
+        case ST_OBJECT_TYPE_CAVITY:
+        {
+            double v, f, lag;
+            std::memcpy( &v, &e[ 0 ], 8 ); std::memcpy( &f, &e[ 1 ], 8 );
+            std::memcpy( &lag, &e[ 2 ], 8 );
+            // be_cavity/track.h:46-57: DEG2RAD = PI/180.0, K = (2 PI)/c
+            double const deg2rad = ST_PI / 180.0;
+            double const kf = ( 2.0 * ST_PI ) / ST_C_LIGHT;
+            rec( ST_OP_CAVITY, 0, { as_u64( v ), as_u64( deg2rad * lag ), as_u64( kf * f ) } );
+            break;
+        }
+        case ST_OBJECT_TYPE_XYSHIFT:
+        {
+            double dx, dy; std::memcpy( &dx, &e[ 0 ], 8 ); std::memcpy( &dy, &e[ 1 ], 8 );
+            rec( ST_OP_XYSHIFT, 0, { as_u64( -dx ), as_u64( -dy ) } );
+            break;
+        }
+        case ST_OBJECT_TYPE_SROTATION: rec( ST_OP_SROTATION, 0, { e[ 0 ], e[ 1 ] } ); break;
+        case ST_OBJECT_TYPE_LIMIT_RECT: rec( ST_OP_LIMIT_RECT, 0, { e[ 0 ], e[ 1 ], e[ 2 ], e[ 3 ] } ); break;
+        case ST_OBJECT_TYPE_LIMIT_ELLIPSE: rec( ST_OP_LIMIT_ELLIPSE, 0, { e[ 0 ], e[ 1 ], e[ 2 ] } ); break;
+        case ST_OBJECT_TYPE_LIMIT_RECT_ELLIPSE:
+            rec( ST_OP_LIMIT_RECT_ELLIPSE, 0, { e[ 0 ], e[ 1 ], e[ 2 ], e[ 3 ], e[ 4 ] } ); break;
+        case ST_OBJECT_TYPE_DIPEDGE: rec( ST_OP_DIPEDGE, 0, { e[ 0 ], e[ 1 ] } ); break;
+        case ST_OBJECT_TYPE_BEAM_MONITOR:
+        {
+            // out address is patched with the device address of the output
+            // particle set by the job (0 = monitor inactive)
+            monitor_fields.push_back( static_cast< uint32_t >( w.size() + 4u ) );
+            rec( ST_OP_MONITOR, 0, { e[ 0 ], e[ 1 ], e[ 2 ], 0u, e[ 4 ], e[ 5 ], e[ 6 ], e[ 7 ] } );
+            break;
+        }
+        case ST_OBJECT_TYPE_BEAM_BEAM_4D:
+        {
+            if( e[ 0 ] < 11u || !inside( e[ 1 ], 11u * 8u ) ) return false;
+            rec( ST_OP_BB4D, 0, std::vector< uint64_t >( at( e[ 1 ] ), at( e[ 1 ] ) + 11 ) );
+            break;
+        }
+        case ST_OBJECT_TYPE_BEAM_BEAM_6D:
+        {
+            if( e[ 0 ] < 38u || !inside( e[ 1 ], 38u * 8u ) ) return false;
+            const uint64_t* d = at( e[ 1 ] );
+            int64_t ns; std::memcpy( &ns, &d[ 18 ], 8 );
+            if( ns < 0 || ns > 4096 ) return false;
+            std::vector< uint64_t > pl( d, d + 38 );
+            // slice arrays: SIXTRL_BB_GET_PTR (be_beamfields.h:23-27)
+            // = &field + field + 1 words; re-based to follow the 38 words
+            for( int j = 0; j < 4; ++j )
+            {
+                uint64_t const src = e[ 1 ] + 8u * ( 34u + j + d[ 34 + j ] + 1u );
+                if( !inside( src, 8u * static_cast< uint64_t >( ns ) ) ) return false;
+                pl.insert( pl.end(), at( src ), at( src ) + ns );
+                pl[ 34 + j ] = ( 38u + j * static_cast< uint64_t >( ns ) ) - ( 34u + j ) - 1u;
+            }
+            rec( ST_OP_BB6D, 0, pl );
+            break;
+        }
+        case ST_OBJECT_TYPE_SC_COASTING:
+            rec( ST_OP_SC_COASTING, 0, std::vector< uint64_t >( e, e + 9 ) ); break;
+        case ST_OBJECT_TYPE_SC_QGAUSSIAN_PROF:
+            rec( ST_OP_SC_QGAUSS, 0, std::vector< uint64_t >( e, e + 11 ) ); break;
+        case ST_OBJECT_TYPE_SC_INTERPOLATED_PROF:
+        {
+            std::vector< uint64_t > pl( e, e + 10 );
+            // line-density table inlined when the profile object lies in
+            // this buffer (be_beamfields.h:887-896); otherwise method 255
+            // ("none") selects the fallback density like a missing profile
+            uint64_t const ld = e[ 6 ];
+            bool ok = ld != 0u && inside( ld, 56u );
+            const uint64_t* L = ok ? at( ld ) : nullptr;
+            int64_t nv = 0;
+            if( ok ) { std::memcpy( &nv, &L[ 6 ], 8 );
+                       ok = nv > 0 && nv < ( 1 << 24 ) && inside( L[ 2 ], 8u * nv ) && inside( L[ 3 ], 8u * nv ); }
+            if( ok )
+            {
+                pl.push_back( L[ 0 ] ); pl.push_back( static_cast< uint64_t >( nv ) );
+                pl.push_back( L[ 4 ] ); pl.push_back( L[ 5 ] );
+                pl.insert( pl.end(), at( L[ 2 ] ), at( L[ 2 ] ) + nv );
+                pl.insert( pl.end(), at( L[ 3 ] ), at( L[ 3 ] ) + nv );
+            }
+            else
+            {
+                pl.push_back( 255u ); pl.push_back( 0u ); pl.push_back( 0u ); pl.push_back( 0u );
+            }
+            rec( ST_OP_SC_INTERPOL, 0, pl );
+            break;
+        }
+        case ST_OBJECT_TYPE_TRICUB:
+        case ST_OBJECT_TYPE_TRICUB_DATA:
+            rec( ST_OP_FAIL, 0, {} ); break;    // track.h:418-451: map disabled
+        default:
+            rec( ST_OP_UNKNOWN, 0, {} ); break; // track.h:577-581
+        }
+        return true;
+    }
+};
+}  // namespace
+
+extern "C" int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,
+    int64_t out_capacity, uint32_t* rec_offsets, int64_t* n_elem_out,
+    uint32_t* monitor_fields, int64_t* n_monitors_out )
+{
+    if( belems == nullptr ) return -1;
+    const uint64_t* h = reinterpret_cast< const uint64_t* >( belems );
+    if( h[ 0 ] != reinterpret_cast< uint64_t >( belems ) ) return -1;   // not relocated
+    int64_t n = 0;
+    const Obj* obj = objects( belems, &n );
+    Packer pk;
+    pk.lo = h[ 0 ];
+    pk.hi = h[ 0 ] + h[ 1 ];
+    pk.offs.reserve( n + 1 );
+    for( int64_t i = 0; i < n; ++i )
+    {
+        pk.offs.push_back( static_cast< uint32_t >( pk.w.size() ) );
+        if( !pk.add( obj[ i ] ) ) return -1;
+    }
+    pk.offs.push_back( static_cast< uint32_t >( pk.w.size() ) );
+    int64_t const nw = static_cast< int64_t >( pk.w.size() );
+    if( n_elem_out ) *n_elem_out = n;
+    if( n_monitors_out ) *n_monitors_out = static_cast< int64_t >( pk.monitor_fields.size() );
+    if( out != nullptr )
+    {
+        if( out_capacity < nw ) return -1;
+        std::memcpy( out, pk.w.data(), sizeof( uint64_t ) * nw );
+        if( rec_offsets ) std::memcpy( rec_offsets, pk.offs.data(), sizeof( uint32_t ) * pk.offs.size() );
+        if( monitor_fields && !pk.monitor_fields.empty() )
+            std::memcpy( monitor_fields, pk.monitor_fields.data(), sizeof( uint32_t ) * pk.monitor_fields.size() );
+    }
+    return nw;
+}

@@ -1,0 +1,655 @@
+// NS(TrackJob) on HIP: host side of the drop-in boundary.
+//
+// Reference behaviour followed:
+//   construction / reset      common/internal/track_job_base.cpp:2107-2437
+//   output preparation        common/output/output_buffer.c:44-363,
+//                             be_monitor/output_buffer.h:96-140,
+//                             output/elem_by_elem_config.h:1050-1110
+//   track / collect / push    common/internal/track_job.cpp:417-446,
+//                             cuda/track/track_job.cpp:2321-2390
+// Design (this project's): the particle and output buffers live on the
+// device as byte images of the host CObjects buffers relocated to the
+// device base address, so collect is one copy + one host relocation; the
+// lattice is flattened once into the packed stream of lattice.h; every
+// launch runs on a job-owned stream; track_* return after the launch has
+// finished and the OR'ed status has been read back.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cctype>
+#include <climits>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sixtracklib_amd/track_job.h"
+#include "sixtracklib_amd/beam_elements.h"
+#include "sixtracklib_amd/lattice_pack.h"
+#include "lattice.h"
+#include "track_launch.h"
+
+namespace
+{
+thread_local std::string g_err;
+
+bool hip_ok( hipError_t e, const char* what )
+{
+    if( e == hipSuccess ) return true;
+    g_err = std::string( what ) + ": " + hipGetErrorString( e );
+    return false;
+}
+
+bool is_element_type( uint64_t t )
+{
+    return t != ST_OBJECT_TYPE_NONE && t != ST_OBJECT_TYPE_PARTICLE &&
+           t != ST_OBJECT_TYPE_INVALID && t != ST_OBJECT_TYPE_LINE &&
+           t != ST_OBJECT_TYPE_ELEM_BY_ELEM_CONF;
+}
+
+struct Attr
+{
+    int64_t min_pid = LLONG_MAX, max_pid = LLONG_MIN;
+    int64_t min_elem = LLONG_MAX, max_elem = LLONG_MIN;
+    int64_t min_turn = LLONG_MAX, max_turn = LLONG_MIN;
+};
+
+// output/output_buffer.h:461-560 (particle extrema folded with the
+// element-id range of the lattice, elem_by_elem_output_buffer.h:163-235)
+bool find_attributes( NS(Buffer)* pbuf, const std::vector< uint64_t >& sets,
+                      NS(Buffer)* ebuf, Attr& a, int64_t* n_ebe_objs )
+{
+    for( uint64_t k : sets )
+    {
+        NS(Particles) const* p = NS(Particles_buffer_get_const_particles)( pbuf, k );
+        if( p == nullptr ) return false;
+        NS(Particles_find_min_max_attributes)( p, &a.min_pid, &a.max_pid,
+            &a.min_elem, &a.max_elem, &a.min_turn, &a.max_turn );
+    }
+    if( a.min_pid == LLONG_MAX ) { a.min_pid = a.max_pid = 0; a.min_elem = a.max_elem = 0;
+                                   a.min_turn = a.max_turn = 0; }
+    if( a.min_pid < 0 ) a.min_pid = 0;
+    int64_t const n = ( int64_t )NS(Buffer_get_num_of_objects)( ebuf );
+    int64_t lo = 0, count = 0;
+    int64_t i = 0;
+    for( ; i < n; ++i )
+    {
+        if( is_element_type( NS(Buffer_get_const_object)( ebuf, i )->type_id ) ) { ++count; ++i; break; }
+        ++lo;
+    }
+    int64_t hi = lo;
+    for( ; i < n; ++i )
+        if( is_element_type( NS(Buffer_get_const_object)( ebuf, i )->type_id ) ) { ++count; ++hi; }
+    if( count == 0 ) lo = hi = 0;
+    a.min_elem = std::min( a.min_elem, lo );
+    a.max_elem = std::max( a.max_elem, hi );
+    if( n_ebe_objs ) *n_ebe_objs = count;
+    return true;
+}
+
+struct OutputPlan
+{
+    bool has_ebe = false;
+    int64_t ebe_size = 0, max_ebe_turn = 0;
+    std::vector< int64_t > monitor_elems;      // element index of every monitor
+    std::vector< int64_t > monitor_sizes;
+};
+
+// output/output_buffer.c:288-363
+void plan_output( NS(Buffer)* ebuf, const Attr& a, uint64_t until_turn_ebe, OutputPlan& o )
+{
+    int64_t const np = a.max_pid - a.min_pid + 1;
+    if( a.min_turn >= 0 && ( int64_t )until_turn_ebe > a.min_turn )
+    {
+        o.has_ebe = true;
+        o.max_ebe_turn = ( int64_t )until_turn_ebe - 1;
+        o.ebe_size = np * ( a.max_elem - a.min_elem + 1 ) * ( o.max_ebe_turn - a.min_turn + 1 );
+    }
+    int64_t const n = ( int64_t )NS(Buffer_get_num_of_objects)( ebuf );
+    for( int64_t i = 0; i < n; ++i )
+    {
+        NS(Object) const* obj = NS(Buffer_get_const_object)( ebuf, i );
+        if( obj->type_id != ST_OBJECT_TYPE_BEAM_MONITOR ) continue;
+        NS(BeamMonitor) const* m = ( NS(BeamMonitor) const* )( uintptr_t )obj->begin_addr;
+        if( m->num_stores <= 0 ) continue;
+        o.monitor_elems.push_back( i );
+        o.monitor_sizes.push_back( m->num_stores * np );
+    }
+}
+}  // namespace
+
+struct NS(TrackJobBase)
+{
+    std::string arch = "hip", config, device_id;
+    NS(Buffer)* pbuf = nullptr;
+    NS(Buffer)* ebuf = nullptr;
+    NS(Buffer)* obuf = nullptr;
+    bool owns_obuf = false;
+    std::vector< uint64_t > sets;
+    std::vector< int64_t > set_sizes;
+    Attr attr;
+    OutputPlan plan;
+    uint64_t ebe_offset = 0, monitor_offset = 0, until_turn_ebe = 0;
+    st_ebe_config_t ebe_cfg{};
+    bool ebe_rolling = true, fma = false;
+    int block = 256, device = 0;
+    uint16_t collect_flags = ST_TRACK_JOB_COLLECT_DEFAULT_FLAGS;
+    int32_t last_status = 0;
+
+    hipStream_t stream = nullptr;
+    unsigned char* d_pbuf = nullptr; size_t pbuf_bytes = 0;
+    unsigned char* d_obuf = nullptr; size_t obuf_bytes = 0;
+    uint64_t* d_lat = nullptr; int64_t lat_words = 0, n_elem = 0;
+    std::vector< uint32_t > rec_offsets, monitor_fields;
+    std::vector< uint64_t > lat_host;
+    bool lat_full = false;
+    int32_t* d_status = nullptr; int32_t* h_status = nullptr;
+    std::vector< st_dev_particles_t > views;
+    st_dev_particles_t ebe_view{};
+
+    ~NS(TrackJobBase)() { release(); }
+
+    void release()
+    {
+        if( d_pbuf ) hipFree( d_pbuf );
+        if( d_obuf ) hipFree( d_obuf );
+        if( d_lat ) hipFree( d_lat );
+        if( d_status ) hipFree( d_status );
+        if( h_status ) hipHostFree( h_status );
+        if( stream ) hipStreamDestroy( stream );
+        d_pbuf = d_obuf = nullptr; d_lat = nullptr; d_status = nullptr;
+        h_status = nullptr; stream = nullptr;
+        if( owns_obuf && obuf ) NS(Buffer_delete)( obuf );
+        obuf = nullptr; owns_obuf = false;
+    }
+
+    // -- host <-> device images ------------------------------------------
+    bool upload_image( NS(Buffer)* b, unsigned char* d, size_t bytes )
+    {
+        std::vector< uint64_t > staging( ( bytes + 7 ) / 8 );
+        std::memcpy( staging.data(), NS(Buffer_get_const_data_begin)( b ), bytes );
+        if( NS(ManagedBuffer_remap_to)( reinterpret_cast< unsigned char* >( staging.data() ),
+                                        ( NS(buffer_addr_t) )( uintptr_t )d ) != 0 )
+        { g_err = "relocation of buffer image failed"; return false; }
+        return hip_ok( hipMemcpyAsync( d, staging.data(), bytes, hipMemcpyHostToDevice, stream ),
+                       "hipMemcpyAsync H2D" ) &&
+               hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" );
+    }
+
+    bool download_image( NS(Buffer)* b, const unsigned char* d, size_t bytes )
+    {
+        unsigned char* h = NS(Buffer_get_data_begin)( b );
+        if( NS(Buffer_get_size)( b ) != bytes ) { g_err = "host buffer changed size"; return false; }
+        if( !hip_ok( hipMemcpyAsync( h, d, bytes, hipMemcpyDeviceToHost, stream ), "hipMemcpyAsync D2H" ) ||
+            !hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" ) ) return false;
+        return NS(ManagedBuffer_remap_to)( h, 0u ) == 0;
+    }
+
+    st_dev_particles_t device_view( NS(Buffer)* host, unsigned char* d, uint64_t index )
+    {
+        st_dev_particles_t v{};
+        NS(Particles)* hp = NS(Particles_buffer_get_particles)( host, index );
+        uint64_t const hb = NS(Buffer_get_data_begin_addr)( host );
+        v.n = hp->num_particles;
+        double* const* hr = &hp->q0;
+        double** dr = &v.q0;
+        for( int k = 0; k < 17; ++k )
+            dr[ k ] = reinterpret_cast< double* >( d + ( ( uint64_t )( uintptr_t )hr[ k ] - hb ) );
+        int64_t* const* hi = &hp->particle_id;
+        int64_t** di = &v.particle_id;
+        for( int k = 0; k < 4; ++k )
+            di[ k ] = reinterpret_cast< int64_t* >( d + ( ( uint64_t )( uintptr_t )hi[ k ] - hb ) );
+        return v;
+    }
+
+    bool pack_lattice()
+    {
+        int64_t n_elem_l = 0, n_mon = 0;
+        const unsigned char* eb = NS(Buffer_get_const_data_begin)( ebuf );
+        int64_t const nw = NS(Lattice_pack)( eb, nullptr, 0, nullptr, &n_elem_l, nullptr, &n_mon );
+        if( nw < 0 ) { g_err = "lattice could not be packed"; return false; }
+        lat_host.assign( std::max< int64_t >( nw, 1 ), 0u );
+        rec_offsets.assign( n_elem_l + 1, 0u );
+        monitor_fields.assign( n_mon, 0u );
+        if( NS(Lattice_pack)( eb, lat_host.data(), nw, rec_offsets.data(), &n_elem_l,
+                              monitor_fields.data(), &n_mon ) != nw ) return false;
+        n_elem = n_elem_l;
+        lat_words = nw;
+        lat_full = false;
+        for( int64_t i = 0; i < n_elem; ++i )
+        {
+            unsigned const op = ST_REC_OP( lat_host[ rec_offsets[ i ] ] );
+            if( op > ST_OP_DIPEDGE || op == 0 ) lat_full = true;
+        }
+        // point the monitors that have an output set at its device header
+        if( plan.monitor_elems.size() > 0 && d_obuf != nullptr )
+        {
+            uint64_t const hb = NS(Buffer_get_data_begin_addr)( obuf );
+            size_t m = 0;
+            for( int64_t i = 0; i < n_elem && m < plan.monitor_elems.size(); ++i )
+            {
+                if( ST_REC_OP( lat_host[ rec_offsets[ i ] ] ) != ST_OP_MONITOR ) continue;
+                uint32_t const field = rec_offsets[ i ] + 4u;
+                if( i == plan.monitor_elems[ m ] )
+                {
+                    NS(Object) const* o = NS(Buffer_get_const_object)( obuf, monitor_offset + m );
+                    lat_host[ field ] = ( uint64_t )( uintptr_t )d_obuf + ( o->begin_addr - hb );
+                    ++m;
+                }
+            }
+        }
+        if( d_lat ) { hipFree( d_lat ); d_lat = nullptr; }
+        if( !hip_ok( hipMalloc( &d_lat, lat_host.size() * 8u ), "hipMalloc lattice" ) ) return false;
+        return hip_ok( hipMemcpyAsync( d_lat, lat_host.data(), lat_host.size() * 8u,
+                                       hipMemcpyHostToDevice, stream ), "upload lattice" ) &&
+               hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" );
+    }
+
+    bool prepare_output()
+    {
+        std::vector< int64_t > need;
+        if( plan.has_ebe ) need.push_back( plan.ebe_size );
+        need.insert( need.end(), plan.monitor_sizes.begin(), plan.monitor_sizes.end() );
+        if( need.empty() ) return true;
+        if( obuf == nullptr ) { obuf = NS(Buffer_new)( 0 ); owns_obuf = true; }
+        if( obuf == nullptr ) { g_err = "cannot allocate output buffer"; return false; }
+        // reuse a caller-prepared buffer whose leading sets match exactly
+        uint64_t base = NS(Buffer_get_num_of_objects)( obuf );
+        bool match = base >= need.size();
+        for( size_t k = 0; match && k < need.size(); ++k )
+        {
+            NS(Particles) const* p = NS(Particles_buffer_get_const_particles)( obuf, k );
+            match = p != nullptr && p->num_particles == need[ k ];
+        }
+        if( match ) base = 0;
+        else
+            for( int64_t sz : need )
+                if( NS(Particles_new)( obuf, sz ) == nullptr )
+                { g_err = "output buffer cannot hold the required particle sets"; return false; }
+        ebe_offset = base;
+        monitor_offset = base + ( plan.has_ebe ? 1u : 0u );
+        // be_monitor/output_buffer.h:96-140: out_address and particle range
+        for( size_t m = 0; m < plan.monitor_elems.size(); ++m )
+        {
+            NS(Object)* eo = NS(Buffer_get_object)( ebuf, plan.monitor_elems[ m ] );
+            NS(BeamMonitor)* mon = ( NS(BeamMonitor)* )( uintptr_t )eo->begin_addr;
+            mon->min_particle_id = attr.min_pid;
+            mon->max_particle_id = attr.max_pid;
+            mon->out_address = NS(Buffer_get_const_object)( obuf, monitor_offset + m )->begin_addr;
+        }
+        if( plan.has_ebe )
+        {
+            st_ebe_config_t& c = ebe_cfg;
+            c.order = 0;
+            c.num_particles_to_store = attr.max_pid - attr.min_pid + 1;
+            c.num_elements_to_store = attr.max_elem - attr.min_elem + 1;
+            c.num_turns_to_store = plan.max_ebe_turn - attr.min_turn + 1;
+            c.min_particle_id = attr.min_pid; c.max_particle_id = attr.max_pid;
+            c.min_element_id = attr.min_elem; c.max_element_id = attr.max_elem;
+            c.min_turn = attr.min_turn; c.max_turn = plan.max_ebe_turn;
+            c.is_rolling = ebe_rolling ? 1 : 0;
+            c.out_store_addr = NS(Buffer_get_const_object)( obuf, ebe_offset )->begin_addr;
+        }
+        return true;
+    }
+
+    bool setup( NS(Buffer)* p, uint64_t nsets, const uint64_t* idx, NS(Buffer)* e,
+                NS(Buffer)* o, uint64_t ute )
+    {
+        release();
+        if( p == nullptr || e == nullptr ) { g_err = "particles and beam elements buffers required"; return false; }
+        if( NS(Buffer_needs_remapping)( p ) || NS(Buffer_needs_remapping)( e ) ||
+            ( o != nullptr && NS(Buffer_needs_remapping)( o ) ) )
+        { g_err = "buffers must not need remapping"; return false; }
+        pbuf = p; ebuf = e; obuf = o; owns_obuf = false; until_turn_ebe = ute;
+        sets.clear(); set_sizes.clear();
+        if( nsets == 0 || idx == nullptr ) sets.push_back( 0u );
+        else sets.assign( idx, idx + nsets );
+        for( uint64_t k : sets )
+        {
+            NS(Particles) const* pp = NS(Particles_buffer_get_const_particles)( pbuf, k );
+            if( pp == nullptr ) { g_err = "particle set index does not name a Particles object"; return false; }
+            set_sizes.push_back( pp->num_particles );
+        }
+        attr = Attr();
+        plan = OutputPlan();
+        if( !find_attributes( pbuf, sets, ebuf, attr, nullptr ) ) return false;
+        plan_output( ebuf, attr, until_turn_ebe, plan );
+        if( !prepare_output() ) return false;
+
+        if( !hip_ok( hipSetDevice( device ), "hipSetDevice" ) ) return false;
+        if( !hip_ok( hipStreamCreate( &stream ), "hipStreamCreate" ) ) return false;
+        if( !hip_ok( hipMalloc( &d_status, sizeof( int32_t ) ), "hipMalloc status" ) ) return false;
+        if( !hip_ok( hipHostMalloc( &h_status, sizeof( int32_t ) ), "hipHostMalloc" ) ) return false;
+        pbuf_bytes = NS(Buffer_get_size)( pbuf );
+        if( !hip_ok( hipMalloc( &d_pbuf, pbuf_bytes ), "hipMalloc particles" ) ) return false;
+        if( !upload_image( pbuf, d_pbuf, pbuf_bytes ) ) return false;
+        if( obuf != nullptr )
+        {
+            obuf_bytes = NS(Buffer_get_size)( obuf );
+            if( !hip_ok( hipMalloc( &d_obuf, obuf_bytes ), "hipMalloc output" ) ) return false;
+            if( !upload_image( obuf, d_obuf, obuf_bytes ) ) return false;
+        }
+        if( !pack_lattice() ) return false;
+        views.clear();
+        for( uint64_t k : sets ) views.push_back( device_view( pbuf, d_pbuf, k ) );
+        if( plan.has_ebe ) ebe_view = device_view( obuf, d_obuf, ebe_offset );
+        return true;
+    }
+
+    bool begin_status()
+    {
+        return hip_ok( hipMemsetAsync( d_status, 0, sizeof( int32_t ), stream ), "hipMemsetAsync" );
+    }
+
+    int32_t end_status( bool launched_ok )
+    {
+        if( !launched_ok ) { last_status = -1; return -1; }
+        if( !hip_ok( hipMemcpyAsync( h_status, d_status, sizeof( int32_t ),
+                                     hipMemcpyDeviceToHost, stream ), "status D2H" ) ||
+            !hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" ) )
+        { last_status = -1; return -1; }
+        last_status = *h_status;
+        return last_status;
+    }
+
+    int32_t track_until( int64_t until_turn )
+    {
+        if( !begin_status() ) return end_status( false );
+        bool ok = true;
+        for( const st_dev_particles_t& v : views )
+        {
+            hipError_t const err = fma
+                ? st_launch_track_until_fma( &v, d_lat, n_elem, until_turn, lat_full, d_status, block, stream )
+                : st_launch_track_until_exact( &v, d_lat, n_elem, until_turn, lat_full, d_status, block, stream );
+            ok = ok && hip_ok( err, "launch track_until" );
+        }
+        return end_status( ok );
+    }
+
+    int32_t track_ebe( int64_t until_turn )
+    {
+        if( !plan.has_ebe ) return track_until( until_turn );
+        if( !begin_status() ) return end_status( false );
+        bool ok = true;
+        for( const st_dev_particles_t& v : views )
+        {
+            hipError_t const err = fma
+                ? st_launch_track_ebe_fma( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), &ebe_view, d_status, block, stream )
+                : st_launch_track_ebe_exact( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), &ebe_view, d_status, block, stream );
+            ok = ok && hip_ok( err, "launch track_elem_by_elem" );
+        }
+        return end_status( ok );
+    }
+
+    st_ebe_config_t dev_cfg{};
+    const st_ebe_config_t& ebe_device_cfg()
+    {
+        dev_cfg = ebe_cfg;
+        dev_cfg.out_store_addr = ( uint64_t )( uintptr_t )d_obuf +
+            ( ebe_cfg.out_store_addr - NS(Buffer_get_data_begin_addr)( obuf ) );
+        return dev_cfg;
+    }
+
+    int32_t track_line( uint64_t b, uint64_t e, bool finish )
+    {
+        if( b > e || e > ( uint64_t )n_elem ) { last_status = -1; return -1; }
+        if( !begin_status() ) return end_status( false );
+        bool ok = true;
+        const uint64_t* lat = d_lat + rec_offsets[ b ];
+        for( const st_dev_particles_t& v : views )
+        {
+            hipError_t const err = fma
+                ? st_launch_track_line_fma( &v, lat, ( int64_t )( e - b ), finish, d_status, block, stream )
+                : st_launch_track_line_exact( &v, lat, ( int64_t )( e - b ), finish, d_status, block, stream );
+            ok = ok && hip_ok( err, "launch track_line" );
+        }
+        return end_status( ok );
+    }
+
+    void parse_config( const char* cfg )
+    {
+        if( cfg == nullptr ) return;
+        config = cfg;
+        std::string s = cfg;
+        for( char& ch : s ) if( ch == ',' || ch == ';' ) ch = ' ';
+        size_t pos = 0;
+        while( pos < s.size() )
+        {
+            size_t const end = s.find( ' ', pos );
+            std::string const tok = s.substr( pos, end == std::string::npos ? std::string::npos : end - pos );
+            pos = ( end == std::string::npos ) ? s.size() : end + 1;
+            size_t const eq = tok.find( '=' );
+            if( eq == std::string::npos ) continue;
+            std::string const k = tok.substr( 0, eq ), v = tok.substr( eq + 1 );
+            long const iv = std::strtol( v.c_str(), nullptr, 10 );
+            if( k == "fma" ) fma = iv != 0;
+            else if( k == "block" ) block = ( int )iv;
+            else if( k == "device" ) { device = ( int )iv; device_id = v; }
+            else if( k == "ebe_rolling" ) ebe_rolling = iv != 0;
+        }
+    }
+};
+
+namespace
+{
+bool parse_arch( const char* arch, NS(TrackJobBase)* job )
+{
+    if( arch == nullptr ) return false;
+    std::string a = arch;
+    for( char& c : a ) c = ( char )std::tolower( ( unsigned char )c );
+    size_t const colon = a.find( ':' );
+    if( colon != std::string::npos )
+    {
+        job->device_id = a.substr( colon + 1 );
+        job->device = ( int )std::strtol( job->device_id.c_str(), nullptr, 10 );
+        a = a.substr( 0, colon );
+    }
+    return a == "hip" || a == "amd" || a == "amdgpu" || a == "gpu";
+}
+
+NS(TrackJobBase)* make_job( const char* arch, const char* config )
+{
+    NS(TrackJobBase)* job = new NS(TrackJobBase)();
+    if( !parse_arch( arch, job ) )
+    {
+        g_err = std::string( "unsupported architecture '" ) + ( arch ? arch : "(null)" ) +
+                "' (this library provides \"hip\" only)";
+        delete job;
+        return nullptr;
+    }
+    job->parse_config( config );
+    return job;
+}
+}  // namespace
+
+extern "C" {
+
+NS(TrackJobBase)* NS(TrackJob_create)( char const* arch, char const* config_str )
+{
+    return make_job( arch, config_str );
+}
+
+NS(TrackJobBase)* NS(TrackJob_new_detailed)( char const* arch, NS(Buffer)* pbuf,
+    NS(buffer_size_t) nsets, NS(buffer_size_t) const* idx, NS(Buffer)* ebuf,
+    NS(Buffer)* obuf, NS(buffer_size_t) until_turn_ebe, char const* config_str )
+{
+    NS(TrackJobBase)* job = make_job( arch, config_str );
+    if( job == nullptr ) return nullptr;
+    if( !job->setup( pbuf, nsets, idx, ebuf, obuf, until_turn_ebe ) ) { delete job; return nullptr; }
+    return job;
+}
+
+NS(TrackJobBase)* NS(TrackJob_new_with_output)( char const* arch, NS(Buffer)* pbuf,
+    NS(Buffer)* ebuf, NS(Buffer)* obuf, NS(buffer_size_t) until_turn_ebe,
+    char const* config_str )
+{
+    return NS(TrackJob_new_detailed)( arch, pbuf, 0u, nullptr, ebuf, obuf, until_turn_ebe, config_str );
+}
+
+NS(TrackJobBase)* NS(TrackJob_new)( char const* arch, NS(Buffer)* pbuf,
+    NS(Buffer)* ebuf, char const* config_str )
+{
+    return NS(TrackJob_new_detailed)( arch, pbuf, 0u, nullptr, ebuf, nullptr, 0u, config_str );
+}
+
+void NS(TrackJob_delete)( NS(TrackJobBase)* job ) { delete job; }
+
+NS(track_status_t) NS(TrackJob_track_until)( NS(TrackJobBase)* job, NS(buffer_size_t) until_turn )
+{
+    return ( job && job->d_pbuf ) ? job->track_until( ( int64_t )until_turn ) : -1;
+}
+
+NS(track_status_t) NS(TrackJob_track_elem_by_elem)( NS(TrackJobBase)* job, NS(buffer_size_t) until_turn )
+{
+    return ( job && job->d_pbuf ) ? job->track_ebe( ( int64_t )until_turn ) : -1;
+}
+
+NS(track_status_t) NS(TrackJob_track_line)( NS(TrackJobBase)* job, NS(buffer_size_t) b,
+                                            NS(buffer_size_t) e, bool finish_turn )
+{
+    return ( job && job->d_pbuf ) ? job->track_line( b, e, finish_turn ) : -1;
+}
+
+void NS(TrackJob_collect_detailed)( NS(TrackJobBase)* job, NS(track_job_collect_flag_t) flags )
+{
+    if( job == nullptr || job->d_pbuf == nullptr ) return;
+    if( flags & ST_TRACK_JOB_IO_PARTICLES )
+        job->download_image( job->pbuf, job->d_pbuf, job->pbuf_bytes );
+    if( ( flags & ST_TRACK_JOB_IO_OUTPUT ) && job->obuf && job->d_obuf )
+        job->download_image( job->obuf, job->d_obuf, job->obuf_bytes );
+    /* beam elements are never modified on the device: nothing to collect */
+}
+
+void NS(TrackJob_collect)( NS(TrackJobBase)* job )
+{
+    if( job ) NS(TrackJob_collect_detailed)( job, job->collect_flags );
+}
+void NS(TrackJob_collect_particles)( NS(TrackJobBase)* job ) { NS(TrackJob_collect_detailed)( job, ST_TRACK_JOB_IO_PARTICLES ); }
+void NS(TrackJob_collect_beam_elements)( NS(TrackJobBase)* job ) { NS(TrackJob_collect_detailed)( job, ST_TRACK_JOB_IO_BEAM_ELEMENTS ); }
+void NS(TrackJob_collect_output)( NS(TrackJobBase)* job ) { NS(TrackJob_collect_detailed)( job, ST_TRACK_JOB_IO_OUTPUT ); }
+
+static void set_flag( NS(TrackJobBase)* job, uint16_t f, bool on )
+{
+    if( job ) job->collect_flags = on ? ( job->collect_flags | f ) : ( job->collect_flags & ~f );
+}
+void NS(TrackJob_enable_collect_particles)( NS(TrackJobBase)* j ) { set_flag( j, ST_TRACK_JOB_IO_PARTICLES, true ); }
+void NS(TrackJob_disable_collect_particles)( NS(TrackJobBase)* j ) { set_flag( j, ST_TRACK_JOB_IO_PARTICLES, false ); }
+bool NS(TrackJob_is_collecting_particles)( const NS(TrackJobBase)* j ) { return j && ( j->collect_flags & ST_TRACK_JOB_IO_PARTICLES ); }
+void NS(TrackJob_enable_collect_beam_elements)( NS(TrackJobBase)* j ) { set_flag( j, ST_TRACK_JOB_IO_BEAM_ELEMENTS, true ); }
+void NS(TrackJob_disable_collect_beam_elements)( NS(TrackJobBase)* j ) { set_flag( j, ST_TRACK_JOB_IO_BEAM_ELEMENTS, false ); }
+bool NS(TrackJob_is_collecting_beam_elements)( const NS(TrackJobBase)* j ) { return j && ( j->collect_flags & ST_TRACK_JOB_IO_BEAM_ELEMENTS ); }
+void NS(TrackJob_enable_collect_output)( NS(TrackJobBase)* j ) { set_flag( j, ST_TRACK_JOB_IO_OUTPUT, true ); }
+void NS(TrackJob_disable_collect_output)( NS(TrackJobBase)* j ) { set_flag( j, ST_TRACK_JOB_IO_OUTPUT, false ); }
+bool NS(TrackJob_is_collecting_output)( const NS(TrackJobBase)* j ) { return j && ( j->collect_flags & ST_TRACK_JOB_IO_OUTPUT ); }
+NS(track_job_collect_flag_t) NS(TrackJob_get_collect_flags)( const NS(TrackJobBase)* j ) { return j ? j->collect_flags : 0; }
+void NS(TrackJob_set_collect_flags)( NS(TrackJobBase)* j, NS(track_job_collect_flag_t) f ) { if( j ) j->collect_flags = f; }
+bool NS(TrackJob_requires_collecting)( const NS(TrackJobBase)* j ) { return j != nullptr; }
+
+void NS(TrackJob_push)( NS(TrackJobBase)* job, NS(track_job_push_flag_t) flags )
+{
+    if( job == nullptr || job->d_pbuf == nullptr ) return;
+    if( flags & ST_TRACK_JOB_IO_PARTICLES ) job->upload_image( job->pbuf, job->d_pbuf, job->pbuf_bytes );
+    if( flags & ST_TRACK_JOB_IO_BEAM_ELEMENTS ) job->pack_lattice();
+    if( ( flags & ST_TRACK_JOB_IO_OUTPUT ) && job->obuf && job->d_obuf )
+        job->upload_image( job->obuf, job->d_obuf, job->obuf_bytes );
+}
+void NS(TrackJob_push_particles)( NS(TrackJobBase)* j ) { NS(TrackJob_push)( j, ST_TRACK_JOB_IO_PARTICLES ); }
+void NS(TrackJob_push_beam_elements)( NS(TrackJobBase)* j ) { NS(TrackJob_push)( j, ST_TRACK_JOB_IO_BEAM_ELEMENTS ); }
+void NS(TrackJob_push_output)( NS(TrackJobBase)* j ) { NS(TrackJob_push)( j, ST_TRACK_JOB_IO_OUTPUT ); }
+
+void NS(TrackJob_clear)( NS(TrackJobBase)* job ) { if( job ) job->release(); }
+
+bool NS(TrackJob_reset_detailed)( NS(TrackJobBase)* job, NS(Buffer)* pbuf,
+    NS(buffer_size_t) nsets, NS(buffer_size_t) const* idx, NS(Buffer)* ebuf,
+    NS(Buffer)* obuf, NS(buffer_size_t) ute )
+{
+    return job != nullptr && job->setup( pbuf, nsets, idx, ebuf, obuf, ute );
+}
+bool NS(TrackJob_reset_with_output)( NS(TrackJobBase)* job, NS(Buffer)* pbuf,
+    NS(Buffer)* ebuf, NS(Buffer)* obuf, NS(buffer_size_t) ute )
+{
+    return NS(TrackJob_reset_detailed)( job, pbuf, 0u, nullptr, ebuf, obuf, ute );
+}
+bool NS(TrackJob_reset)( NS(TrackJobBase)* job, NS(Buffer)* pbuf, NS(Buffer)* ebuf, NS(Buffer)* obuf )
+{
+    return NS(TrackJob_reset_detailed)( job, pbuf, 0u, nullptr, ebuf, obuf, 0u );
+}
+bool NS(TrackJob_select_particle_set)( NS(TrackJobBase)* job, NS(buffer_size_t) k )
+{
+    if( job == nullptr ) return false;
+    return job->setup( job->pbuf, 1u, &k, job->ebuf, job->owns_obuf ? nullptr : job->obuf,
+                       job->until_turn_ebe );
+}
+bool NS(TrackJob_assign_output_buffer)( NS(TrackJobBase)* job, NS(Buffer)* obuf )
+{
+    if( job == nullptr ) return false;
+    std::vector< uint64_t > s = job->sets;
+    return job->setup( job->pbuf, s.size(), s.data(), job->ebuf, obuf, job->until_turn_ebe );
+}
+
+NS(track_job_type_t) NS(TrackJob_get_type_id)( const NS(TrackJobBase)* j ) { return j ? 3 : -1; }
+char const* NS(TrackJob_get_type_str)( const NS(TrackJobBase)* j ) { return j ? j->arch.c_str() : ""; }
+bool NS(TrackJob_has_device_id_str)( const NS(TrackJobBase)* j ) { return j && !j->device_id.empty(); }
+char const* NS(TrackJob_get_device_id_str)( const NS(TrackJobBase)* j ) { return j ? j->device_id.c_str() : ""; }
+bool NS(TrackJob_has_config_str)( const NS(TrackJobBase)* j ) { return j && !j->config.empty(); }
+char const* NS(TrackJob_get_config_str)( const NS(TrackJobBase)* j ) { return j ? j->config.c_str() : ""; }
+NS(buffer_size_t) NS(TrackJob_get_num_particle_sets)( const NS(TrackJobBase)* j ) { return j ? j->sets.size() : 0u; }
+NS(buffer_size_t) const* NS(TrackJob_get_particle_set_indices_begin)( const NS(TrackJobBase)* j ) { return j ? j->sets.data() : nullptr; }
+NS(buffer_size_t) const* NS(TrackJob_get_particle_set_indices_end)( const NS(TrackJobBase)* j ) { return j ? j->sets.data() + j->sets.size() : nullptr; }
+NS(buffer_size_t) NS(TrackJob_get_particle_set_index)( const NS(TrackJobBase)* j, NS(buffer_size_t) n ) { return ( j && n < j->sets.size() ) ? j->sets[ n ] : 0u; }
+NS(buffer_size_t) NS(TrackJob_get_total_num_particles_in_sets)( const NS(TrackJobBase)* j )
+{
+    NS(buffer_size_t) t = 0u;
+    if( j ) for( int64_t n : j->set_sizes ) t += ( NS(buffer_size_t) )n;
+    return t;
+}
+NS(particle_index_t) NS(TrackJob_get_min_particle_id)( const NS(TrackJobBase)* j ) { return j ? j->attr.min_pid : -1; }
+NS(particle_index_t) NS(TrackJob_get_max_particle_id)( const NS(TrackJobBase)* j ) { return j ? j->attr.max_pid : -1; }
+NS(particle_index_t) NS(TrackJob_get_min_element_id)( const NS(TrackJobBase)* j ) { return j ? j->attr.min_elem : -1; }
+NS(particle_index_t) NS(TrackJob_get_max_element_id)( const NS(TrackJobBase)* j ) { return j ? j->attr.max_elem : -1; }
+NS(particle_index_t) NS(TrackJob_get_min_initial_turn_id)( const NS(TrackJobBase)* j ) { return j ? j->attr.min_turn : -1; }
+NS(particle_index_t) NS(TrackJob_get_max_initial_turn_id)( const NS(TrackJobBase)* j ) { return j ? j->attr.max_turn : -1; }
+NS(Buffer)* NS(TrackJob_get_particles_buffer)( NS(TrackJobBase)* j ) { return j ? j->pbuf : nullptr; }
+NS(Buffer)* NS(TrackJob_get_beam_elements_buffer)( NS(TrackJobBase)* j ) { return j ? j->ebuf : nullptr; }
+bool NS(TrackJob_has_output_buffer)( const NS(TrackJobBase)* j ) { return j && j->obuf != nullptr; }
+bool NS(TrackJob_owns_output_buffer)( const NS(TrackJobBase)* j ) { return j && j->owns_obuf; }
+bool NS(TrackJob_has_elem_by_elem_output)( const NS(TrackJobBase)* j ) { return j && j->plan.has_ebe; }
+bool NS(TrackJob_has_beam_monitor_output)( const NS(TrackJobBase)* j ) { return j && !j->plan.monitor_elems.empty(); }
+NS(buffer_size_t) NS(TrackJob_get_beam_monitor_output_buffer_offset)( const NS(TrackJobBase)* j ) { return j ? j->monitor_offset : 0u; }
+NS(buffer_size_t) NS(TrackJob_get_elem_by_elem_output_buffer_offset)( const NS(TrackJobBase)* j ) { return j ? j->ebe_offset : 0u; }
+NS(buffer_size_t) NS(TrackJob_get_num_elem_by_elem_turns)( const NS(TrackJobBase)* j ) { return j ? j->until_turn_ebe : 0u; }
+NS(buffer_size_t) NS(TrackJob_get_num_beam_monitors)( const NS(TrackJobBase)* j ) { return j ? j->plan.monitor_elems.size() : 0u; }
+NS(Buffer)* NS(TrackJob_get_output_buffer)( NS(TrackJobBase)* j ) { return j ? j->obuf : nullptr; }
+NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* j ) { return j ? j->last_status : -1; }
+
+int64_t NS(TrackJob_required_output_sets)( NS(Buffer)* pbuf, NS(buffer_size_t) nsets,
+    NS(buffer_size_t) const* idx, NS(Buffer)* ebuf, NS(buffer_size_t) ute,
+    int64_t* out_sizes, int64_t max_sets, int64_t* has_ebe )
+{
+    if( pbuf == nullptr || ebuf == nullptr ) return -1;
+    std::vector< uint64_t > sets;
+    if( nsets == 0 || idx == nullptr ) sets.push_back( 0u ); else sets.assign( idx, idx + nsets );
+    Attr a;
+    OutputPlan plan;
+    if( !find_attributes( pbuf, sets, ebuf, a, nullptr ) ) return -1;
+    plan_output( ebuf, a, ute, plan );
+    std::vector< int64_t > need;
+    if( plan.has_ebe ) need.push_back( plan.ebe_size );
+    need.insert( need.end(), plan.monitor_sizes.begin(), plan.monitor_sizes.end() );
+    if( has_ebe ) *has_ebe = plan.has_ebe ? 1 : 0;
+    for( int64_t k = 0; k < ( int64_t )need.size() && k < max_sets && out_sizes; ++k )
+        out_sizes[ k ] = need[ k ];
+    return ( int64_t )need.size();
+}
+
+int NS(Hip_num_devices)( void )
+{
+    int n = 0;
+    if( hipGetDeviceCount( &n ) != hipSuccess ) return 0;
+    return n;
+}
+
+char const* NS(Hip_last_error_str)( void ) { return g_err.c_str(); }
+
+}  // extern "C"

@@ -1,0 +1,294 @@
+"""Synthetic lattices and particle sets for tests and benchmarks.
+
+All generators are seeded and deterministic.  Workloads (sizes from the
+benchmark definitions of this project):
+
+* ``fodo_ring``        -- 100 FODO cells [Drift 1 m, thin quad knl[1] =
+  +-0.01, Drift 1 m, thin sextupole knl[2] = 0.05] + Cavity(12 MV, 400 MHz,
+  lag 180 deg) + LimitRect(+-0.05 m): 402 elements.
+* ``lhc_like_ring``    -- 20,000-element ring built from 400 cells of 50
+  elements (27 drifts L~U(0.1, 10) m, 20 thin multipoles of order 0..10:
+  focusing/defocusing quads sized for a 72 deg FODO phase advance plus
+  sextupoles and random orders, all with N(0, 1e-4) relative errors,
+  2 XYShift/SRotation with N(0, 1e-4) offsets/angles, 1 LimitEllipse),
+  with 8 of the cells carrying a cavity in place of one drift.
+* ``kitchen_sink``     -- short lattice containing every element type the
+  trackers support (parity tests).
+* particle sets: Gaussian beams, polar amplitude grids.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from .cobjects import CBuffer
+from .particles import Particles, update_delta_arrays, PROTON_MASS_EV
+from . import beam_elements as be
+
+
+# ---------------------------------------------------------------- particles
+
+def make_particles(n, p0c=6.5e12, mass0=PROTON_MASS_EV, q0=1.0, x=0.0, px=0.0,
+                   y=0.0, py=0.0, zeta=0.0, delta=0.0, cbuffer=None,
+                   particle_id=None, at_turn=0, at_element=0, state=1):
+    """A particle set with the energy variables derived exactly as the C
+    tracker's NS(Particles_update_delta) (particles.h:5459-5497)."""
+    cb = cbuffer if cbuffer is not None else CBuffer(
+        max_slots=22 * int(n) + 64, max_objects=4, max_pointers=24)
+    p = Particles(cbuffer=cb, num_particles=int(n))
+    energy0 = math.sqrt(p0c * p0c + mass0 * mass0)
+    beta0 = p0c / energy0
+    gamma0 = energy0 / mass0
+    view = lambda name: type(p).__dict__[name]._view(p)
+    view("q0")[:] = q0
+    view("mass0")[:] = mass0
+    view("p0c")[:] = p0c
+    view("beta0")[:] = beta0
+    view("gamma0")[:] = gamma0
+    view("x")[:] = x
+    view("px")[:] = px
+    view("y")[:] = y
+    view("py")[:] = py
+    view("zeta")[:] = zeta
+    d = np.broadcast_to(np.asarray(delta, dtype=np.float64), (int(n),)).copy()
+    rvv, rpp, psigma = update_delta_arrays(d, np.full(int(n), beta0))
+    view("delta")[:] = d
+    view("rvv")[:] = rvv
+    view("rpp")[:] = rpp
+    view("psigma")[:] = psigma
+    view("chi")[:] = 1.0
+    view("charge_ratio")[:] = 1.0
+    view("s")[:] = 0.0
+    view("particle_id")[:] = np.arange(n) if particle_id is None else particle_id
+    view("at_element")[:] = at_element
+    view("at_turn")[:] = at_turn
+    view("state")[:] = state
+    return cb, p
+
+
+def gaussian_beam(n, seed=0, sigma_x=1e-3, sigma_px=1e-5, sigma_y=1e-3,
+                  sigma_py=1e-5, sigma_zeta=0.075, sigma_delta=1e-4, **kw):
+    """x, px, y, py, zeta, delta ~ N(0, sigma) drawn in that order from
+    numpy default_rng(seed)."""
+    rng = np.random.default_rng(seed)
+    x = rng.normal(0.0, sigma_x, n)
+    px = rng.normal(0.0, sigma_px, n)
+    y = rng.normal(0.0, sigma_y, n)
+    py = rng.normal(0.0, sigma_py, n)
+    zeta = rng.normal(0.0, sigma_zeta, n)
+    delta = rng.normal(0.0, sigma_delta, n)
+    return make_particles(n, x=x, px=px, y=y, py=py, zeta=zeta, delta=delta, **kw)
+
+
+def polar_grid(n_amplitudes, n_angles, sigma_x, sigma_y, max_sigma=20.0,
+               delta=2.7e-4, **kw):
+    """Particles on a polar grid of normalised amplitude r in (0, max_sigma]
+    sigma and angle theta in [0, pi/2): x = r sigma_x cos(theta),
+    y = r sigma_y sin(theta), px = py = 0, amplitude-major order (a wave of
+    64 lanes holds one amplitude at 64 angles)."""
+    r = np.linspace(max_sigma / n_amplitudes, max_sigma, n_amplitudes)
+    th = (np.arange(n_angles) + 0.5) * (0.5 * np.pi / n_angles)
+    rr, tt = np.meshgrid(r, th, indexing="ij")
+    x = (rr * np.cos(tt) * sigma_x).ravel()
+    y = (rr * np.sin(tt) * sigma_y).ravel()
+    return make_particles(x.size, x=x, y=y, delta=delta, **kw)
+
+
+# ----------------------------------------------------------------- lattices
+
+def fodo_ring(n_cells=100, k1=0.01, k2=0.05, drift=1.0, voltage=12e6,
+              frequency=400e6, lag=180.0, limit=0.05, cbuffer=None):
+    eb = cbuffer if cbuffer is not None else CBuffer(
+        max_slots=8 * (4 * n_cells + 2) + 64, max_objects=4 * n_cells + 8,
+        max_pointers=2 * n_cells + 8)
+    for c in range(n_cells):
+        be.Drift(cbuffer=eb, length=drift)
+        be.Multipole(cbuffer=eb, knl=[0.0, k1 if c % 2 == 0 else -k1])
+        be.Drift(cbuffer=eb, length=drift)
+        be.Multipole(cbuffer=eb, knl=[0.0, 0.0, k2])
+    be.Cavity(cbuffer=eb, voltage=voltage, frequency=frequency, lag=lag)
+    be.LimitRect(cbuffer=eb, min_x=-limit, max_x=limit, min_y=-limit, max_y=limit)
+    return eb
+
+
+def lhc_like_ring(seed=0, n_cells=400, cell_len=50, mu_deg=72.0, rel_err=1e-4,
+                  k2_scale=0.02, r0=0.017, n_cavities=8, voltage=2e6,
+                  frequency=400e6, ellipse_a=0.02, ellipse_b=0.015,
+                  error_seed=None, cbuffer=None, return_info=False):
+    """20,000-element synthetic ring (see module docstring).  ``seed`` fixes
+    the structure (drift lengths, element orders, placements);
+    ``error_seed`` (default ``seed``) the multipole error table, so machine
+    seeds of one ring differ only in their errors."""
+    if error_seed is None:
+        error_seed = seed
+    rng = np.random.default_rng(seed)
+    erng = np.random.default_rng(10_000 + int(error_seed))
+    n_elem = n_cells * cell_len
+    eb = cbuffer if cbuffer is not None else CBuffer(
+        max_slots=n_elem * 16, max_objects=n_elem + 16, max_pointers=n_elem)
+    cav_cells = set(np.linspace(0, n_cells - 1, n_cavities).astype(int).tolist())
+    n_drift, n_mp, n_misc = 27, 20, 2
+    counts = {"drift": 0, "multipole": 0, "xyshift": 0, "srotation": 0,
+              "ellipse": 0, "cavity": 0}
+    orders = []
+    sin_half = math.sin(math.radians(mu_deg) / 2.0)
+    for c in range(n_cells):
+        lengths = rng.uniform(0.1, 10.0, n_drift)
+        # slot layout: QF, then half of the other slots, QD, the rest; the
+        # ellipse is the cell's last element (every cell_len-th element)
+        others = ["drift"] * n_drift + ["mp"] * (n_mp - 2) + ["misc"] * n_misc
+        rng.shuffle(others)
+        half = len(others) // 2
+        seq = ["QF"] + others[:half] + ["QD"] + others[half:] + ["ellipse"]
+        assert len(seq) == cell_len
+        # focal length from the cell's own half-cell lengths (thin FODO)
+        di = iter(range(n_drift))
+        dl = [lengths[next(di)] if s == "drift" else 0.0 for s in seq]
+        l1 = sum(dl[: seq.index("QD")])
+        l2 = sum(dl[seq.index("QD"):])
+        lh = 0.5 * (l1 + l2)
+        k1 = 2.0 * sin_half / lh
+        cav_slot = None
+        if c in cav_cells:
+            cav_slot = [i for i, s in enumerate(seq) if s == "drift"][0]
+        di = iter(range(n_drift))
+        misc_toggle = 0
+        for i, s in enumerate(seq):
+            if s == "drift":
+                L = lengths[next(di)]
+                if i == cav_slot:
+                    be.Cavity(cbuffer=eb, voltage=voltage, frequency=frequency,
+                              lag=180.0)
+                    counts["cavity"] += 1
+                else:
+                    be.Drift(cbuffer=eb, length=L)
+                    counts["drift"] += 1
+            elif s in ("QF", "QD", "mp"):
+                if s == "mp":
+                    order = int(rng.integers(0, 11))
+                else:
+                    order = int(rng.integers(1, 11))
+                knl = np.zeros(order + 1)
+                ksl = np.zeros(order + 1)
+                kref = k1
+                if s == "QF":
+                    knl[1] = +k1
+                elif s == "QD":
+                    knl[1] = -k1
+                elif order >= 2 and rng.random() < 0.5:
+                    knl[2] = k2_scale * (1.0 if rng.random() < 0.5 else -1.0)
+                # relative errors: b_n / b_ref at the reference radius r0
+                nn = np.arange(order + 1)
+                scale = rel_err * kref * np.array([math.factorial(int(m)) for m in nn]) \
+                    * r0 ** (1.0 - nn)
+                knl = knl + scale * erng.normal(0.0, 1.0, order + 1)
+                ksl = ksl + scale * erng.normal(0.0, 1.0, order + 1)
+                be.Multipole(cbuffer=eb, knl=knl, ksl=ksl)
+                orders.append(order)
+                counts["multipole"] += 1
+            elif s == "misc":
+                if misc_toggle == 0:
+                    be.XYShift(cbuffer=eb, dx=rng.normal(0, 1e-4), dy=rng.normal(0, 1e-4))
+                    counts["xyshift"] += 1
+                else:
+                    ang = rng.normal(0, 1e-4)
+                    be.SRotation(cbuffer=eb, cos_z=math.cos(ang), sin_z=math.sin(ang))
+                    counts["srotation"] += 1
+                misc_toggle ^= 1
+            elif s == "ellipse":
+                be.LimitEllipse(cbuffer=eb, a=ellipse_a, b=ellipse_b)
+                counts["ellipse"] += 1
+    if return_info:
+        return eb, {"counts": counts, "orders": orders}
+    return eb
+
+
+def flops_per_particle_turn(ebuf):
+    """Algorithmic FP64 operation count of one particle-turn through
+    ``ebuf`` (+ - * / sqrt = 1, transcendental calls counted separately):
+    Drift 15 + 2 aperture compares, Multipole 8*order + 4 (+ curvature),
+    XYShift 2, SRotation 12, LimitEllipse 5, Cavity 37 + 1 sin,
+    DriftExact 17 + 2, LimitRect 0, DipoleEdge 4."""
+    from .cobject import get_object
+    fl = 0
+    trans = 0
+    for i in range(ebuf.n_objects):
+        t = ebuf.get_object_typeid(i)
+        if t == 2:
+            fl += 17
+        elif t == 3:
+            fl += 19
+        elif t == 4:
+            o = get_object(ebuf, i)
+            fl += 8 * int(o.order) + 4
+            if o.hxl != 0.0 or o.hyl != 0.0:
+                fl += 12
+        elif t == 5:
+            fl += 37
+            trans += 1
+        elif t == 6:
+            fl += 2
+        elif t == 7:
+            fl += 12
+        elif t == 12:
+            fl += 5
+        elif t == 16:
+            fl += 7
+        elif t == 24:
+            fl += 4
+    return fl, trans
+
+
+def twiss_sigma(beta=100.0, emit_norm=2.5e-6, p0c=6.5e12, mass0=PROTON_MASS_EV):
+    """rms beam size sqrt(beta * eps_n / (beta_rel gamma_rel))."""
+    bg = p0c / mass0
+    return math.sqrt(beta * emit_norm / bg)
+
+
+def kitchen_sink(seed=0, cbuffer=None, with_monitor=False):
+    """Every supported element type once or more, moderate strengths."""
+    rng = np.random.default_rng(seed)
+    eb = cbuffer if cbuffer is not None else CBuffer(max_slots=4096, max_objects=128,
+                                                     max_pointers=64)
+    be.Drift(cbuffer=eb, length=0.5)
+    be.Multipole(cbuffer=eb, knl=[1e-5, 0.02, 0.1, 1.0], ksl=[0.0, 1e-4, 0.0, 0.5])
+    be.DriftExact(cbuffer=eb, length=0.7)
+    be.Multipole(cbuffer=eb, knl=[0.001, -0.01], hxl=0.001, hyl=0.0005, length=1.5)
+    be.XYShift(cbuffer=eb, dx=1e-4, dy=-2e-4)
+    be.SRotation(cbuffer=eb, angle=3.0)
+    be.DipoleEdge(cbuffer=eb, r21=0.002, r43=-0.003)
+    be.Cavity(cbuffer=eb, voltage=5e6, frequency=200e6, lag=170.0)
+    be.RFMultipole(cbuffer=eb, voltage=1e5, frequency=400e6, lag=10.0,
+                   knl=[1e-6, 1e-3, 0.01], ksl=[0.0, 2e-4, 0.0], pn=[0.0, 30.0, 60.0],
+                   ps=[10.0, 0.0, -40.0])
+    be.LimitRect(cbuffer=eb, min_x=-0.02, max_x=0.02, min_y=-0.015, max_y=0.015)
+    be.Drift(cbuffer=eb, length=1.0)
+    be.BeamBeam4D(cbuffer=eb, charge=1e11, sigma_x=2e-3, sigma_y=1.5e-3, beta_r=1.0,
+                  min_sigma_diff=1e-10, x_bb=1e-4, y_bb=-1e-4, d_px=0.0, d_py=0.0,
+                  enabled=1.0)
+    be.BeamBeam4D(cbuffer=eb, charge=1e11, sigma_x=2e-3, sigma_y=2e-3, beta_r=1.0,
+                  min_sigma_diff=1e-10, x_bb=0.0, y_bb=0.0, d_px=1e-7, d_py=0.0,
+                  enabled=1.0)
+    be.BeamBeam6D(cbuffer=eb, phi=0.01, alpha=0.3, x_bb_co=1e-4, y_bb_co=0.0,
+                  charge_slices=[3e10, 4e10, 3e10], zeta_slices=[-0.05, 0.0, 0.05],
+                  sigma_11=4e-6, sigma_12=1e-8, sigma_13=0.0, sigma_14=0.0,
+                  sigma_22=1e-9, sigma_23=0.0, sigma_24=0.0, sigma_33=2.5e-6,
+                  sigma_34=-1e-8, sigma_44=1e-9, x_co=0.0, px_co=0.0, y_co=0.0,
+                  py_co=0.0, zeta_co=0.0, delta_co=0.0, min_sigma_diff=1e-10,
+                  threshold_singular=1e-28, d_x=0.0, d_px=0.0, d_y=0.0, d_py=0.0,
+                  d_zeta=0.0, d_delta=0.0, enabled=1)
+    be.LimitEllipse(cbuffer=eb, a=0.02, b=0.02)
+    be.SCCoasting(cbuffer=eb, number_of_particles=1e11, circumference=1000.0,
+                  sigma_x=1.2e-3, sigma_y=0.9e-3, length=1.0, x_co=0.0, y_co=0.0)
+    be.SCQGaussProfile(cbuffer=eb, number_of_particles=1e11, bunchlength_rms=0.1,
+                       sigma_x=1.1e-3, sigma_y=1.1e-3, length=1.0, q_param=1.2)
+    be.SCQGaussProfile(cbuffer=eb, number_of_particles=1e11, bunchlength_rms=0.1,
+                       sigma_x=1.3e-3, sigma_y=0.8e-3, length=1.0, q_param=1.0)
+    be.SCInterpolatedProfile(cbuffer=eb, number_of_particles=1e11, sigma_x=1e-3,
+                             sigma_y=1.2e-3, length=1.0, line_density_prof_fallback=0.5)
+    be.LimitRectEllipse(cbuffer=eb, max_x=0.02, max_y=0.02, a_squ=4e-4, b_squ=3e-4)
+    if with_monitor:
+        be.BeamMonitor(cbuffer=eb, num_stores=4, start=0, skip=1, is_rolling=1)
+    be.Drift(cbuffer=eb, length=0.3)
+    return eb

@@ -1,0 +1,131 @@
+"""ctypes bindings to libsixtrack_amd.so.
+
+Counterpart of python/sixtracklib/stcommon.py (``ct.CDLL(stconf.SHARED_LIB)``
+and the ``st_*`` prototypes, e.g. :1334-1375 for the track job).  The
+library is loaded from the package directory; if it is missing this module
+raises ImportError -- there is no pure-Python fallback for tracking.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SHARED_LIB = os.path.join(HERE, "libsixtrack_amd.so")
+
+if not os.path.exists(SHARED_LIB):
+    raise ImportError("libsixtrack_amd.so not built: run `python -m sixtracklib_amd.build` "
+                      "(or __graft_entry__.build())")
+
+sixtracklib = ct.CDLL(SHARED_LIB)
+
+st_buffer_size_t = ct.c_uint64
+st_buffer_addr_t = ct.c_uint64
+st_track_status_t = ct.c_int32
+st_particle_index_t = ct.c_int64
+st_Buffer_p = ct.c_void_p
+st_NullBuffer = None
+st_TrackJob_p = ct.c_void_p
+st_NullTrackJob = None
+st_uchar_p = ct.c_void_p
+st_buffer_size_p = ct.POINTER(st_buffer_size_t)
+
+st_TRACK_SUCCESS = 0
+st_TRACK_STATUS_GENERAL_FAILURE = -1
+st_TRACK_JOB_IO_PARTICLES = 0x01
+st_TRACK_JOB_IO_BEAM_ELEMENTS = 0x02
+st_TRACK_JOB_IO_OUTPUT = 0x04
+
+
+def _fn(name, restype, argtypes):
+    f = getattr(sixtracklib, name)
+    f.restype = restype
+    f.argtypes = argtypes
+    globals()[name] = f
+    return f
+
+
+# -- buffers ------------------------------------------------------------
+_fn("st_Buffer_new", st_Buffer_p, [st_buffer_size_t])
+_fn("st_Buffer_new_detailed", st_Buffer_p, [st_buffer_size_t] * 4 + [ct.c_uint64])
+_fn("st_Buffer_new_from_file", st_Buffer_p, [ct.c_char_p])
+_fn("st_Buffer_new_on_memory", st_Buffer_p, [st_uchar_p, st_buffer_size_t])
+_fn("st_Buffer_new_mapped_on_cbuffer", st_Buffer_p, [st_uchar_p, st_buffer_size_t])
+_fn("st_Buffer_new_from_copy", st_Buffer_p, [st_Buffer_p])
+_fn("st_Buffer_delete", None, [st_Buffer_p])
+_fn("st_Buffer_write_to_file", ct.c_bool, [st_Buffer_p, ct.c_char_p])
+_fn("st_Buffer_needs_remapping", ct.c_bool, [st_Buffer_p])
+_fn("st_Buffer_remap", ct.c_int, [st_Buffer_p])
+_fn("st_Buffer_get_num_of_objects", st_buffer_size_t, [st_Buffer_p])
+_fn("st_Buffer_get_size", st_buffer_size_t, [st_Buffer_p])
+_fn("st_Buffer_get_capacity", st_buffer_size_t, [st_Buffer_p])
+_fn("st_Buffer_get_data_begin_addr", st_buffer_addr_t, [st_Buffer_p])
+_fn("st_Buffer_get_object", ct.c_void_p, [st_Buffer_p, st_buffer_size_t])
+_fn("st_ManagedBuffer_remap_to", ct.c_int, [st_uchar_p, st_buffer_addr_t])
+
+# -- particles / elements -------------------------------------------------
+_fn("st_Particles_new", ct.c_void_p, [st_Buffer_p, ct.c_int64])
+_fn("st_Particles_buffer_get_particles", ct.c_void_p, [st_Buffer_p, st_buffer_size_t])
+_fn("st_Particles_buffer_get_num_of_particle_blocks", st_buffer_size_t, [st_Buffer_p])
+_fn("st_Particles_buffer_get_total_num_of_particles", ct.c_int64, [st_Buffer_p])
+_fn("st_Drift_add", ct.c_void_p, [st_Buffer_p, ct.c_double])
+_fn("st_DriftExact_add", ct.c_void_p, [st_Buffer_p, ct.c_double])
+_fn("st_Multipole_add", ct.c_void_p, [st_Buffer_p, ct.c_int64, ct.c_double,
+                                      ct.c_double, ct.c_double, st_buffer_addr_t])
+_fn("st_Cavity_add", ct.c_void_p, [st_Buffer_p, ct.c_double, ct.c_double, ct.c_double])
+_fn("st_LimitRect_add", ct.c_void_p, [st_Buffer_p] + [ct.c_double] * 4)
+_fn("st_Math_q_gauss_cq", ct.c_double, [ct.c_double])
+_fn("st_Lattice_pack", ct.c_int64, [st_uchar_p, ct.c_void_p, ct.c_int64, ct.c_void_p,
+                                    ct.POINTER(ct.c_int64), ct.c_void_p,
+                                    ct.POINTER(ct.c_int64)])
+
+# -- track job ----------------------------------------------------------
+_fn("st_TrackJob_create", st_TrackJob_p, [ct.c_char_p, ct.c_char_p])
+_fn("st_TrackJob_new", st_TrackJob_p, [ct.c_char_p, st_Buffer_p, st_Buffer_p, ct.c_char_p])
+_fn("st_TrackJob_new_with_output", st_TrackJob_p,
+    [ct.c_char_p, st_Buffer_p, st_Buffer_p, st_Buffer_p, st_buffer_size_t, ct.c_char_p])
+_fn("st_TrackJob_new_detailed", st_TrackJob_p,
+    [ct.c_char_p, st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p,
+     st_Buffer_p, st_buffer_size_t, ct.c_char_p])
+_fn("st_TrackJob_delete", None, [st_TrackJob_p])
+_fn("st_TrackJob_track_until", st_track_status_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_track_elem_by_elem", st_track_status_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_track_line", st_track_status_t,
+    [st_TrackJob_p, st_buffer_size_t, st_buffer_size_t, ct.c_bool])
+for _n in ("collect", "collect_particles", "collect_beam_elements", "collect_output",
+           "push_particles", "push_beam_elements", "push_output", "clear",
+           "enable_collect_particles", "disable_collect_particles",
+           "enable_collect_output", "disable_collect_output"):
+    _fn("st_TrackJob_" + _n, None, [st_TrackJob_p])
+_fn("st_TrackJob_collect_detailed", None, [st_TrackJob_p, ct.c_uint16])
+_fn("st_TrackJob_push", None, [st_TrackJob_p, ct.c_uint16])
+for _n in ("has_output_buffer", "owns_output_buffer", "has_elem_by_elem_output",
+           "has_beam_monitor_output", "has_device_id_str", "has_config_str",
+           "requires_collecting", "is_collecting_particles", "is_collecting_output"):
+    _fn("st_TrackJob_" + _n, ct.c_bool, [st_TrackJob_p])
+for _n in ("get_beam_monitor_output_buffer_offset", "get_elem_by_elem_output_buffer_offset",
+           "get_num_elem_by_elem_turns", "get_num_beam_monitors", "get_num_particle_sets",
+           "get_total_num_particles_in_sets"):
+    _fn("st_TrackJob_" + _n, st_buffer_size_t, [st_TrackJob_p])
+for _n in ("get_min_particle_id", "get_max_particle_id", "get_min_element_id",
+           "get_max_element_id", "get_min_initial_turn_id", "get_max_initial_turn_id"):
+    _fn("st_TrackJob_" + _n, st_particle_index_t, [st_TrackJob_p])
+_fn("st_TrackJob_get_type_str", ct.c_char_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_config_str", ct.c_char_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_device_id_str", ct.c_char_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_output_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_last_status", st_track_status_t, [st_TrackJob_p])
+_fn("st_TrackJob_required_output_sets", ct.c_int64,
+    [st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p, st_buffer_size_t,
+     ct.POINTER(ct.c_int64), ct.c_int64, ct.POINTER(ct.c_int64)])
+_fn("st_Hip_num_devices", ct.c_int, [])
+_fn("st_Hip_last_error_str", ct.c_char_p, [])
+
+
+def string_to_encoded_ctypes_str(s):
+    return None if s is None else s.encode("utf-8")
+
+
+def last_error():
+    msg = st_Hip_last_error_str()
+    return msg.decode() if msg else ""

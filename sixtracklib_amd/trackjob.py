@@ -1,0 +1,229 @@
+"""Python ``TrackJob``: drop-in for ``sixtracklib.TrackJob``
+(python/sixtracklib/trackjob.py:1213-1560).
+
+Same constructor signature and methods (``track_until``,
+``track_elem_by_elem``, ``track_line``, ``collect``, ``collect_particles``,
+``collect_output``, ``push_particles``, ``push_beam_elements``, ``output``,
+``has_elem_by_elem_output``, ...).  Buffers are this package's
+:class:`~sixtracklib_amd.cobjects.CBuffer` (or objects carrying one:
+``Elements``, ``ParticlesSet``, ``Particles``); as in the reference, the
+job maps the buffers' memory, so ``collect()`` updates the Python-side
+arrays in place.  Tracking runs on the GPU only: ``arch`` is "hip"; the
+reference's "cpu"/"opencl" are rejected with ``ValueError``.
+"""
+from __future__ import annotations
+
+import ctypes as ct
+
+import numpy as np
+
+from . import stcommon as st
+from .cobjects import CBuffer
+from .particles import Particles
+
+_GPU_ARCHS = ("hip", "amd", "amdgpu", "gpu")
+
+
+def get_cbuffer_from_obj(obj):
+    """python/sixtracklib/buffer.py get_cbuffer_from_obj"""
+    if isinstance(obj, CBuffer):
+        return obj
+    for attr in ("cbuffer", "_buffer"):
+        cb = getattr(obj, attr, None)
+        if isinstance(cb, CBuffer):
+            return cb
+    raise TypeError("cannot get a CBuffer from %r" % (obj,))
+
+
+class TrackJob:
+    def __init__(self, beam_elements_buffer, particles_buffer,
+                 until_turn_elem_by_elem=0, arch="hip", device_id=None,
+                 device=None, output_buffer=None, config_str=None,
+                 particle_set_indices=None):
+        self.ptr_st_track_job = None
+        self._c_buffers = []
+        if device is not None:
+            parts = str(device).split(":", 1)
+            arch = parts[0]
+            if len(parts) > 1:
+                device_id = parts[1]
+        arch = str(arch).strip().lower()
+        if arch not in _GPU_ARCHS:
+            raise ValueError("architecture %r is not available: sixtracklib_amd "
+                             "tracks on AMD GPUs only (arch='hip')" % arch)
+        cfg = config_str or ""
+        if device_id is not None:
+            cfg = (cfg + " device=%s" % str(device_id).split(".")[0]).strip()
+        self._arch = arch
+        self._beam_elements_buffer = get_cbuffer_from_obj(beam_elements_buffer)
+        self._particles_buffer = get_cbuffer_from_obj(particles_buffer)
+        self._until_turn_ebe = int(until_turn_elem_by_elem)
+
+        self._c_pbuf = self._map(self._particles_buffer)
+        self._c_ebuf = self._map(self._beam_elements_buffer)
+
+        if particle_set_indices is None:
+            particle_set_indices = [0]
+        idx = (st.st_buffer_size_t * len(particle_set_indices))(*particle_set_indices)
+        self._set_indices = idx
+
+        # size and build the output buffer on the Python side, like the
+        # reference binding (trackjob.py:1355-1405)
+        self._output_buffer = None
+        self._c_obuf = None
+        sizes = (ct.c_int64 * 4096)()
+        has_ebe = ct.c_int64(0)
+        n = st.st_TrackJob_required_output_sets(
+            self._c_pbuf, len(particle_set_indices), idx, self._c_ebuf,
+            self._until_turn_ebe, sizes, 4096, ct.byref(has_ebe))
+        if n < 0:
+            raise RuntimeError("cannot determine the output requirements: " + st.last_error())
+        if output_buffer is not None:
+            self._output_buffer = get_cbuffer_from_obj(output_buffer)
+        elif n > 0:
+            total = sum(int(sizes[k]) for k in range(n))
+            ob = CBuffer(max_slots=22 * n + 21 * total + 64, max_objects=n + 4,
+                         max_pointers=21 * n + 4)
+            for k in range(n):
+                Particles(cbuffer=ob, num_particles=int(sizes[k]))
+            self._output_buffer = ob
+        if self._output_buffer is not None:
+            self._c_obuf = self._map(self._output_buffer)
+
+        self.ptr_st_track_job = st.st_TrackJob_new_detailed(
+            arch.encode(), self._c_pbuf, len(particle_set_indices), idx,
+            self._c_ebuf, self._c_obuf, self._until_turn_ebe,
+            st.string_to_encoded_ctypes_str(cfg))
+        if not self.ptr_st_track_job:
+            self._free_buffers()
+            raise RuntimeError("unable to create TrackJob: " + st.last_error())
+        self._last_status = 0
+        self._last_track_status = 0
+
+    def _map(self, cbuf):
+        p = st.st_Buffer_new_on_memory(cbuf.pointer, cbuf.words.nbytes)
+        if not p:
+            raise RuntimeError("cannot map buffer")
+        self._c_buffers.append(p)
+        return p
+
+    def _free_buffers(self):
+        for p in self._c_buffers:
+            st.st_Buffer_delete(p)
+        self._c_buffers = []
+
+    def __del__(self):
+        if getattr(self, "ptr_st_track_job", None):
+            st.st_TrackJob_delete(self.ptr_st_track_job)
+            self.ptr_st_track_job = None
+        if getattr(self, "_c_buffers", None):
+            self._free_buffers()
+
+    # -- properties -------------------------------------------------------
+    @property
+    def last_status(self):
+        return self._last_status
+
+    @property
+    def last_track_status(self):
+        return self._last_track_status
+
+    @property
+    def arch(self):
+        return self._arch
+
+    arch_str = arch
+
+    @property
+    def output_buffer(self):
+        return self._output_buffer
+
+    @property
+    def output(self):
+        return self._output_buffer
+
+    @property
+    def particles_buffer(self):
+        return self._particles_buffer
+
+    @property
+    def beam_elements_buffer(self):
+        return self._beam_elements_buffer
+
+    @property
+    def type_str(self):
+        return st.st_TrackJob_get_type_str(self.ptr_st_track_job).decode()
+
+    @property
+    def num_beam_monitors(self):
+        return st.st_TrackJob_get_num_beam_monitors(self.ptr_st_track_job)
+
+    @property
+    def has_elem_by_elem_output(self):
+        return st.st_TrackJob_has_elem_by_elem_output(self.ptr_st_track_job)
+
+    @property
+    def has_beam_monitor_output(self):
+        return st.st_TrackJob_has_beam_monitor_output(self.ptr_st_track_job)
+
+    @property
+    def elem_by_elem_output_offset(self):
+        return st.st_TrackJob_get_elem_by_elem_output_buffer_offset(self.ptr_st_track_job)
+
+    @property
+    def beam_monitor_output_offset(self):
+        return st.st_TrackJob_get_beam_monitor_output_buffer_offset(self.ptr_st_track_job)
+
+    @property
+    def has_output_buffer(self):
+        return st.st_TrackJob_has_output_buffer(self.ptr_st_track_job)
+
+    @property
+    def requires_collecting(self):
+        return True
+
+    # -- tracking ----------------------------------------------------------
+    def _check(self, status, what):
+        self._last_track_status = status
+        self._last_status = status
+        if status != 0:
+            raise RuntimeError("%s failed with status %d %s" % (what, status, st.last_error()))
+        return status
+
+    def track(self, until_turn):
+        return self.track_until(until_turn)
+
+    def track_until(self, until_turn):
+        return self._check(st.st_TrackJob_track_until(self.ptr_st_track_job, int(until_turn)),
+                           "track_until")
+
+    def track_elem_by_elem(self, until_turn):
+        return self._check(st.st_TrackJob_track_elem_by_elem(self.ptr_st_track_job,
+                                                             int(until_turn)),
+                           "track_elem_by_elem")
+
+    def track_line(self, begin_idx, end_idx, finish_turn=False):
+        return self._check(st.st_TrackJob_track_line(self.ptr_st_track_job, int(begin_idx),
+                                                     int(end_idx), bool(finish_turn)),
+                           "track_line")
+
+    def collect(self):
+        st.st_TrackJob_collect(self.ptr_st_track_job)
+
+    def collect_particles(self):
+        st.st_TrackJob_collect_particles(self.ptr_st_track_job)
+
+    def collect_beam_elements(self):
+        st.st_TrackJob_collect_beam_elements(self.ptr_st_track_job)
+
+    def collect_output(self):
+        st.st_TrackJob_collect_output(self.ptr_st_track_job)
+
+    def push_particles(self):
+        st.st_TrackJob_push_particles(self.ptr_st_track_job)
+
+    def push_beam_elements(self):
+        st.st_TrackJob_push_beam_elements(self.ptr_st_track_job)
+
+    def _push_output(self):
+        st.st_TrackJob_push_output(self.ptr_st_track_job)

@@ -1,0 +1,148 @@
+"""C-ABI library checks that need no GPU: the library loads, every symbol
+declared in include/sixtracklib_amd/*.h is exported, the host-side buffer
+and element builders produce reference-format buffers, the lattice packer
+flattens every element type, and the track job refuses to run without a
+HIP device (no CPU fallback exists)."""
+import ctypes as ct
+import os
+import re
+
+import numpy as np
+import pytest
+
+from sixtracklib_amd.cobjects import CBuffer
+from sixtracklib_amd.cobject import get_object
+from sixtracklib_amd import lattices, beam_elements as be
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INCLUDE = os.path.join(ROOT, "include", "sixtracklib_amd")
+
+
+def _declared_symbols():
+    names = set()
+    for fn in os.listdir(INCLUDE):
+        if not fn.endswith(".h"):
+            continue
+        txt = open(os.path.join(INCLUDE, fn)).read()
+        for m in re.finditer(r"ST_API[^;{]*?NS\((\w+)\)\s*\(", txt, re.S):
+            names.add("st_" + m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from sixtracklib_amd import stcommon as st
+    names = _declared_symbols()
+    assert len(names) > 80
+    missing = [n for n in sorted(names) if not hasattr(st.sixtracklib, n)]
+    assert not missing, missing
+
+
+def test_c_buffer_roundtrip_with_python_buffer(tmp_path):
+    from sixtracklib_amd import stcommon as st
+    b = st.st_Buffer_new(0)
+    for k in range(50):
+        assert st.st_Drift_add(b, 0.5 + k)
+    assert st.st_Multipole_add(b, 2, 0.0, 0.0, 0.0, 0)
+    assert st.st_Cavity_add(b, 1e6, 4e8, 180.0)
+    assert st.st_LimitRect_add(b, -0.1, 0.1, -0.2, 0.2)
+    assert st.st_Particles_new(b, 17)
+    path = str(tmp_path / "c.bin")
+    assert st.st_Buffer_write_to_file(b, path.encode())
+    st.st_Buffer_delete(b)
+    cb = CBuffer.fromfile(path)
+    assert cb.n_objects == 54
+    assert [get_object(cb, k).length for k in range(50)] == [0.5 + k for k in range(50)]
+    mp = get_object(cb, 50)
+    assert mp.order == 2 and np.all(mp.bal == 0.0) and len(mp.bal) == 6
+    assert get_object(cb, 51).frequency == 4e8
+    assert get_object(cb, 53).num_particles == 17
+    # Python-written buffer loads in C unchanged
+    eb = lattices.kitchen_sink(0)
+    p2 = str(tmp_path / "py.bin")
+    eb.tofile(p2)
+    b2 = st.st_Buffer_new_from_file(p2.encode())
+    assert st.st_Buffer_get_num_of_objects(b2) == eb.n_objects
+    st.st_Buffer_delete(b2)
+
+
+def test_mapped_buffer_is_relocated_in_place():
+    from sixtracklib_amd import stcommon as st
+    eb = lattices.fodo_ring(n_cells=3)
+    raw = CBuffer.frombytes(eb.tobytes())
+    # relocate the image to another base (as if written by another
+    # process), then map it: the mapping must relocate it back in place
+    assert st.st_ManagedBuffer_remap_to(raw.pointer, raw.pointer + 4096) == 0
+    assert int(raw.words[0]) == raw.pointer + 4096
+    b = st.st_Buffer_new_on_memory(raw.pointer, raw.words.nbytes)
+    assert b and not st.st_Buffer_needs_remapping(b)
+    assert st.st_Buffer_get_num_of_objects(b) == eb.n_objects
+    st.st_Buffer_delete(b)
+
+
+def test_lattice_pack_all_element_types():
+    from sixtracklib_amd import stcommon as st
+    eb = lattices.kitchen_sink(0, with_monitor=True)
+    n_el, n_mon = ct.c_int64(0), ct.c_int64(0)
+    nw = st.st_Lattice_pack(eb.pointer, None, 0, None, ct.byref(n_el), None, ct.byref(n_mon))
+    assert nw > 0 and n_el.value == eb.n_objects and n_mon.value == 1
+    out = np.zeros(nw, dtype=np.uint64)
+    offs = np.zeros(n_el.value + 1, dtype=np.uint32)
+    assert st.st_Lattice_pack(eb.pointer, out.ctypes.data, nw, offs.ctypes.data,
+                              ct.byref(n_el), None, ct.byref(n_mon)) == nw
+    ops = [int(out[o]) & 0xff for o in offs[:-1]]
+    lens = [int(out[o]) >> 32 for o in offs[:-1]]
+    assert offs[-1] == nw and all(np.diff(offs) == lens)
+    # opcode per type id (lattice.h)
+    want = {2: 1, 3: 2, 4: 3, 5: 4, 6: 5, 7: 6, 11: 7, 12: 8, 16: 9, 24: 10, 10: 11,
+            8: 12, 9: 13, 13: 14, 14: 15, 257: 16, 256: 17}
+    assert ops == [want[int(t)] for t in eb.typeids()]
+    # cavity folding: (pi/180)*lag and ((2 pi)/c)*f
+    k = list(eb.typeids()).index(5)
+    cav = out[offs[k] + 1: offs[k] + 4].view(np.float64)
+    assert cav[0] == 5e6
+    assert cav[1] == (np.pi / 180.0) * 170.0
+    assert cav[2] == ((2.0 * np.pi) / 299792458.0) * 200e6
+
+
+def test_lattice_pack_rejects_unrelocated_buffer():
+    from sixtracklib_amd import stcommon as st
+    eb = lattices.fodo_ring(n_cells=2)
+    raw = CBuffer.frombytes(eb.tobytes())
+    assert st.st_ManagedBuffer_remap_to(raw.pointer, raw.pointer + 8) == 0
+    assert st.st_Lattice_pack(raw.pointer, None, 0, None, None, None, None) == -1
+
+
+def test_trackjob_rejects_cpu_arch():
+    from sixtracklib_amd.trackjob import TrackJob
+    eb = lattices.fodo_ring(n_cells=2)
+    pb, _ = lattices.gaussian_beam(8)
+    with pytest.raises(ValueError):
+        TrackJob(eb, pb, arch="cpu")
+
+
+def test_trackjob_required_output_sets():
+    from sixtracklib_amd import stcommon as st
+    eb = lattices.fodo_ring(n_cells=2)
+    be.BeamMonitor(cbuffer=eb, num_stores=4, start=0, skip=1)
+    pb, _ = lattices.gaussian_beam(10)
+    cp = st.st_Buffer_new_on_memory(pb.pointer, pb.words.nbytes)
+    ce = st.st_Buffer_new_on_memory(eb.pointer, eb.words.nbytes)
+    sizes = (ct.c_int64 * 8)()
+    has = ct.c_int64(0)
+    n = st.st_TrackJob_required_output_sets(cp, 0, None, ce, 3, sizes, 8, ct.byref(has))
+    assert n == 2 and has.value == 1
+    assert sizes[0] == 10 * eb.n_objects * 3
+    assert sizes[1] == 4 * 10
+    st.st_Buffer_delete(cp)
+    st.st_Buffer_delete(ce)
+
+
+def test_trackjob_without_device_raises():
+    from sixtracklib_amd import stcommon as st
+    if st.st_Hip_num_devices() > 0:
+        pytest.skip("a device is present")
+    from sixtracklib_amd.trackjob import TrackJob
+    eb = lattices.fodo_ring(n_cells=2)
+    pb, _ = lattices.gaussian_beam(8)
+    with pytest.raises(RuntimeError):
+        TrackJob(eb, pb)

@@ -1,0 +1,272 @@
+"""GPU parity: the HIP tracker through the C-ABI / TrackJob against the CPU
+oracle on the same inputs.
+
+* exact build (fma=0): bit-identical to the oracle (stmath build) on every
+  field, for every element type;
+* fma build (fma=1): within 1e-13 relative (per field, floored by the
+  field's scale) after one turn, identical loss turns / state / ids.
+
+Patterned on tests/sixtracklib/cuda/track/test_track_job_track_until_c99.cpp
+(CPU result is the oracle; 1e-14 there) and the reference's own element
+fixtures.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from helpers import particles_from_arrays, particles_dict, assert_particles_equal
+from sixtracklib_amd.cobjects import CBuffer
+from sixtracklib_amd.particles import ALL_FIELDS, REAL_FIELDS
+from sixtracklib_amd import lattices, beam_elements as be
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _job(eb, pb, fma=False, **kw):
+    from sixtracklib_amd.trackjob import TrackJob
+    cfg = "fma=1" if fma else "fma=0"
+    return TrackJob(eb, pb, config_str=cfg, **kw)
+
+
+def _copy(cb):
+    return CBuffer.frombytes(cb.tobytes())
+
+
+def _pdict(cb, idx=0):
+    from sixtracklib_amd.particles import particles_in
+    return particles_dict(particles_in(cb, idx))
+
+
+def _gpu_until(eb, pb, turns, fma=False):
+    job = _job(eb, pb, fma=fma)
+    job.track_until(turns)
+    job.collect()
+    return _pdict(pb)
+
+
+FMA_REL = 1e-13
+SCALE = {"x": 1e-3, "y": 1e-3, "px": 1e-5, "py": 1e-5, "zeta": 1e-2, "delta": 1e-4,
+         "psigma": 1e-4, "s": 1.0}
+
+
+def _assert_fma_close(got, want, what=""):
+    for f in REAL_FIELDS:
+        a, b = np.asarray(got[f]), np.asarray(want[f])
+        tol = FMA_REL * np.maximum(np.abs(b), SCALE.get(f, 0.0))
+        bad = np.abs(a - b) > tol
+        assert not np.any(bad), "%s %s: %r vs %r" % (what, f, a[bad][:3], b[bad][:3])
+    for f in ("particle_id", "at_element", "at_turn", "state"):
+        assert np.array_equal(got[f], want[f]), "%s %s differs" % (what, f)
+
+
+def test_device_present():
+    from sixtracklib_amd import stcommon as st
+    assert st.st_Hip_num_devices() >= 1
+
+
+def test_drift_fixture_bit_exact_on_gpu():
+    g = np.load(os.path.join(GOLDEN, "drift_golden.npz"))
+    eb = CBuffer()
+    for L in g["lengths"]:
+        be.Drift(cbuffer=eb, length=L)
+    pb, _ = particles_from_arrays({f: g["init_" + f] for f in ALL_FIELDS})
+    got = _gpu_until(eb, pb, 1)
+    assert_particles_equal(got, {f: g["final_" + f] for f in ALL_FIELDS}, what="drift")
+
+
+def test_driftexact_fixture_bit_exact_on_gpu():
+    g = np.load(os.path.join(GOLDEN, "driftexact_golden.npz"))
+    eb = CBuffer()
+    for L in g["lengths"]:
+        be.DriftExact(cbuffer=eb, length=L)
+    pb, _ = particles_from_arrays({f: g["init_" + f] for f in ALL_FIELDS})
+    got = _gpu_until(eb, pb, 1)
+    assert_particles_equal(got, {f: g["final_" + f] for f in ALL_FIELDS}, what="driftexact")
+
+
+@pytest.mark.parametrize("turns", [1, 20])
+def test_fodo_ring_exact_bitwise(turns):
+    eb = lattices.fodo_ring()
+    pb, _ = lattices.gaussian_beam(1000, seed=0)
+    ob = _copy(pb)
+    assert oracle_lib.track_until(ob, eb, turns) == 0
+    got = _gpu_until(eb, pb, turns)
+    assert_particles_equal(got, _pdict(ob), what="fodo %d turns" % turns)
+
+
+def test_fodo_ring_fma_one_turn():
+    eb = lattices.fodo_ring()
+    pb, _ = lattices.gaussian_beam(1000, seed=0)
+    ob = _copy(pb)
+    assert oracle_lib.track_until(ob, eb, 1) == 0
+    got = _gpu_until(eb, pb, 1, fma=True)
+    _assert_fma_close(got, _pdict(ob), "fodo fma")
+
+
+def test_fodo_ring_losses_match_oracle():
+    """Large amplitudes: particles lost at the LimitRect and the global
+    aperture; at_element / at_turn / state must match exactly."""
+    eb = lattices.fodo_ring()
+    pb, _ = lattices.gaussian_beam(2048, seed=3, sigma_x=0.03, sigma_y=0.02,
+                                   sigma_px=3e-4, sigma_py=3e-4)
+    ob = _copy(pb)
+    oracle_lib.track_until(ob, eb, 10)
+    got = _gpu_until(eb, pb, 10)
+    want = _pdict(ob)
+    assert np.count_nonzero(want["state"] == 0) > 100
+    assert_particles_equal(got, want, what="fodo losses")
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_kitchen_sink_exact_bitwise(seed):
+    eb = lattices.kitchen_sink(seed)
+    pb, _ = lattices.gaussian_beam(512, seed=seed, sigma_x=2e-3, sigma_y=2e-3,
+                                   sigma_px=1e-4, sigma_py=1e-4, sigma_zeta=0.05)
+    ob = _copy(pb)
+    st_o = oracle_lib.track_until(ob, eb, 5)
+    job = _job(eb, pb)
+    st_g = job.track_until(5)
+    job.collect()
+    assert st_o == st_g == 0
+    assert_particles_equal(_pdict(pb), _pdict(ob), what="kitchen sink")
+
+
+def test_kitchen_sink_fma_one_turn():
+    eb = lattices.kitchen_sink(0)
+    pb, _ = lattices.gaussian_beam(512, seed=5, sigma_x=2e-3, sigma_y=2e-3,
+                                   sigma_px=1e-4, sigma_py=1e-4, sigma_zeta=0.05)
+    ob = _copy(pb)
+    oracle_lib.track_until(ob, eb, 1)
+    got = _gpu_until(eb, pb, 1, fma=True)
+    _assert_fma_close(got, _pdict(ob), "kitchen sink fma")
+
+
+def test_track_line_lost_particle_behaviour():
+    """tests/sixtracklib/common/track/test_track_job_track_line_c99.cpp:
+    237-384: DriftExact 5 m, quad knl[1]=1e-2 L=1, LimitRect +-0.1,
+    DriftExact 5 m; 100 particles x = 0.002 i; 2 turns tracked by lines
+    equal the until-turn result."""
+    def lattice():
+        eb = CBuffer()
+        be.DriftExact(cbuffer=eb, length=5.0)
+        be.Multipole(cbuffer=eb, knl=[0.0, 1e-2], length=1.0)
+        be.LimitRect(cbuffer=eb, min_x=-0.1, max_x=0.1, min_y=-0.1, max_y=0.1)
+        be.DriftExact(cbuffer=eb, length=5.0)
+        return eb
+    x = 0.002 * np.arange(100)
+    pb_line, _ = lattices.make_particles(100, p0c=1e9, x=x, px=1e-3 * np.arange(100) / 100)
+    pb_until = _copy(pb_line)
+    pb_oracle = _copy(pb_line)
+    eb1, eb2 = lattice(), lattice()
+    job = _job(eb1, pb_line)
+    for _ in range(2):
+        job.track_line(0, 2)
+        job.track_line(2, 3)
+        job.track_line(3, 4, finish_turn=True)
+    job.collect()
+    job2 = _job(eb2, pb_until)
+    job2.track_until(2)
+    job2.collect()
+    assert_particles_equal(_pdict(pb_line), _pdict(pb_until), what="line vs until")
+    oracle_lib.track_until(pb_oracle, lattice(), 2)
+    assert_particles_equal(_pdict(pb_until), _pdict(pb_oracle), what="until vs oracle")
+    assert np.count_nonzero(_pdict(pb_until)["state"] == 0) > 0
+
+
+def test_track_line_matches_oracle_segments():
+    eb = lattices.kitchen_sink(1)
+    n = eb.n_objects
+    pb, _ = lattices.gaussian_beam(256, seed=9, sigma_x=2e-3, sigma_y=2e-3)
+    ob = _copy(pb)
+    job = _job(eb, pb)
+    cuts = [0, 3, 7, 12, n]
+    for a, b in zip(cuts[:-1], cuts[1:]):
+        job.track_line(a, b, finish_turn=(b == n))
+        oracle_lib.track_line(ob, eb, a, b, b == n)
+    job.collect()
+    assert_particles_equal(_pdict(pb), _pdict(ob), what="line segments")
+
+
+def test_elem_by_elem_output_matches_oracle():
+    eb = lattices.fodo_ring(n_cells=10)
+    pb, _ = lattices.gaussian_beam(200, seed=4, sigma_x=0.02, sigma_y=0.02)
+    ob = _copy(pb)
+    from sixtracklib_amd.trackjob import TrackJob
+    job = TrackJob(eb, pb, until_turn_elem_by_elem=3, config_str="fma=0")
+    assert job.has_elem_by_elem_output
+    job.track_elem_by_elem(3)
+    job.collect()
+    out = job.output
+    # oracle with its own output set and the same config
+    from sixtracklib_amd.particles import particles_in, Particles
+    n_out = particles_in(out, job.elem_by_elem_output_offset).num_particles
+    ocb = CBuffer(max_slots=22 * n_out + 64)
+    Particles(cbuffer=ocb, num_particles=n_out)
+    cfg = oracle_lib.EbeConfig(order=0, num_particles_to_store=200,
+                               num_elements_to_store=eb.n_objects, num_turns_to_store=3,
+                               min_particle_id=0, min_element_id=0, min_turn=0,
+                               max_particle_id=199, max_element_id=eb.n_objects - 1,
+                               max_turn=2, is_rolling=1,
+                               out_store_addr=ocb.get_object_address(0))
+    oracle_lib.track_elem_by_elem(ob, eb, cfg, 3)
+    assert_particles_equal(_pdict(pb), _pdict(ob), what="ebe particles")
+    assert_particles_equal(_pdict(out, job.elem_by_elem_output_offset), _pdict(ocb, 0),
+                           what="ebe store")
+
+
+def test_beam_monitor_output_matches_oracle():
+    def lattice():
+        eb = lattices.fodo_ring(n_cells=20)
+        be.BeamMonitor(cbuffer=eb, num_stores=5, start=1, skip=2, is_rolling=1,
+                       is_turn_ordered=1)
+        be.BeamMonitor(cbuffer=eb, num_stores=3, start=0, skip=1, is_rolling=0,
+                       is_turn_ordered=0)
+        return eb
+    eb = lattice()
+    pb, _ = lattices.gaussian_beam(300, seed=11, sigma_x=0.02, sigma_y=0.02)
+    ob = _copy(pb)
+    from sixtracklib_amd.trackjob import TrackJob
+    from sixtracklib_amd.particles import Particles
+    from sixtracklib_amd.cobject import get_object
+    job = TrackJob(eb, pb, config_str="fma=0")
+    assert job.has_beam_monitor_output and job.num_beam_monitors == 2
+    job.track_until(12)
+    job.collect()
+    out = job.output
+    # oracle: same monitors pointing at oracle-owned output sets
+    oeb = lattice()
+    ocb = CBuffer(max_slots=22 * 3000 + 64)
+    m_idx = [i for i in range(oeb.n_objects) if oeb.get_object_typeid(i) == 10]
+    for k, i in enumerate(m_idx):
+        m = get_object(oeb, i)
+        Particles(cbuffer=ocb, num_particles=int(m.num_stores) * 300)
+    for k, i in enumerate(m_idx):
+        m = get_object(oeb, i)
+        m.min_particle_id = 0
+        m.max_particle_id = 299
+        m.out_address = ocb.get_object_address(k)
+    oracle_lib.track_until(ob, oeb, 12)
+    assert_particles_equal(_pdict(pb), _pdict(ob), what="monitor particles")
+    off = job.beam_monitor_output_offset
+    for k in range(2):
+        assert_particles_equal(_pdict(out, off + k), _pdict(ocb, k), what="monitor %d" % k)
+
+
+def test_bbsimple_element_by_element_on_gpu():
+    """Reference fixture dump tracked element by element with track_line."""
+    from test_oracle_golden import _bbsimple_lattice
+    g = np.load(os.path.join(GOLDEN, "bbsimple_golden.npz"))
+    eb = _bbsimple_lattice(g)
+    for ii in range(eb.n_objects):
+        start = {f: g["p%d_%s" % (ii, f)] for f in ALL_FIELDS}
+        want = {f: g["p%d_%s" % (ii + 1, f)] for f in ALL_FIELDS}
+        pb, _ = particles_from_arrays(start)
+        job = _job(eb, pb)
+        job.track_line(ii, ii + 1)
+        job.collect()
+        assert_particles_equal(_pdict(pb), want, abs_tol=1e-15, check_index=False,
+                               what="bbsimple element %d" % ii)
