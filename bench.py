@@ -1,0 +1,198 @@
+"""Benchmark: FP64 particle-turns/s through the synthetic 20,000-element ring.
+
+Workload (one GPU): the 20k-element LHC-like ring of
+sixtracklib_amd.lattices.lhc_like_ring(seed=0) and 1,048,576 particles on a
+polar grid (16,384 amplitudes in (0, 20] sigma x 64 angles, delta 2.7e-4),
+tracked 1000 turns with the HIP tracker (TrackJob, FMA build by default).
+
+A step is one track_until call advancing the whole resident particle set by
+``--turns-per-step`` turns (default 100, so the default 10 steps are the
+1000-turn job).  Warmup steps run first on the same particles; the initial
+state is then pushed again so the timed steps cover turns 0..steps*tps.
+Only tracked particle-turns count: sum over particles of (final at_turn -
+initial at_turn).
+
+Prints one JSON line (see the "roofline" and "cpu_baseline" fields):
+roofline achieved = algorithmic FP64 flops of the tracked particle-turns
+(lattices.flops_per_particle_turn) / summed kernel time measured with HIP
+events on the job's stream; cpu_baseline = the C99 oracle with OpenMP on
+the host cores over a bounded sample of the same workload.
+"""
+import argparse
+import ctypes as ct
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector (FMA = 2 flops)
+HBM_PEAK_GBS = 8000.0
+
+
+def sigma_at_start(eb):
+    """rms beam size for eps_n = 2.5 um at the ring start (one-turn beta
+    from the linear map of the lattice is ~236 m; computed on the host
+    from the thin-lens FODO parameters used by the generator)."""
+    from sixtracklib_amd import lattices
+    return lattices.twiss_sigma(beta=236.0)
+
+
+def build_workload(n_particles, seed=0):
+    from sixtracklib_amd import lattices
+    eb = lattices.lhc_like_ring(seed=seed)
+    sig = sigma_at_start(eb)
+    n_ang = 64
+    n_amp = max(1, n_particles // n_ang)
+    pb, p = lattices.polar_grid(n_amp, n_ang, sigma_x=sig, sigma_y=sig, max_sigma=20.0,
+                                delta=2.7e-4)
+    return eb, pb, p
+
+
+def cpu_baseline(eb, pb, budget_s=10.0, sample=8192, turns_per_chunk=2):
+    """Time the oracle (C99, -ffp-contract=off, OpenMP schedule(static)) on
+    an evenly strided sample of the same particles; returns
+    (particle-turns/s, threads, description)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from sixtracklib_amd import lattices
+    from sixtracklib_amd.particles import particles_in
+    p = particles_in(pb)
+    n = p.num_particles
+    idx = np.linspace(0, n - 1, min(sample, n)).astype(np.int64)
+    sb, sp = lattices.make_particles(len(idx), x=np.array(p.x)[idx], y=np.array(p.y)[idx],
+                                     delta=2.7e-4)
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    at0 = np.array(sp.at_turn).copy()
+    t0 = time.perf_counter()
+    turn = 0
+    while True:
+        turn += turns_per_chunk
+        oracle_lib.track_until(sb, eb, turn, threads=threads)
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    tracked = int(np.sum(np.array(sp.at_turn) - at0))
+    return tracked / dt, threads, ("oracle (C99 restatement, OpenMP) on %d of the %d particles "
+                                   "x %d turns of the same ring, %.1f s" % (len(idx), n, turn, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--turns-per-step", type=int, default=100)
+    ap.add_argument("--particles", type=int, default=1 << 20)
+    ap.add_argument("--mode", choices=("fma", "exact"), default="fma")
+    ap.add_argument("--block", type=int, default=256)
+    ap.add_argument("--cpu-budget", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--exact-steps", type=int, default=1,
+                    help="extra timed steps with the exact (no-FMA) build, reported beside")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU only (the tracker shards particles "
+              "across GPUs without communication; multi-GPU runs are not part of it)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    from sixtracklib_amd import lattices
+    from sixtracklib_amd.particles import particles_in
+    from sixtracklib_amd.trackjob import TrackJob
+
+    t_setup = time.perf_counter()
+    eb, pb, _ = build_workload(args.particles)
+    p = particles_in(pb)
+    init_bytes = pb.words.copy()
+    flops_pt, n_trans = lattices.flops_per_particle_turn(eb)
+    tps = args.turns_per_step
+
+    def run(mode, steps, warmup):
+        pb.words[:] = init_bytes
+        job = TrackJob(eb, pb, config_str="fma=%d block=%d" % (mode == "fma", args.block))
+        for w in range(warmup):
+            job.track_until((w + 1) * tps)
+        if warmup:
+            pb.words[:] = init_bytes
+            job.push_particles()
+        at0 = np.array(p.at_turn).copy()
+        kernel_ms = 0.0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            job.track_until((k + 1) * tps)
+            kernel_ms += job.last_kernel_time_ms
+        t1 = time.perf_counter()
+        job.collect_particles()
+        tracked = int(np.sum(np.array(p.at_turn) - at0))
+        alive = int(np.count_nonzero(np.array(p.state) == 1))
+        del job
+        return tracked, t1 - t0, kernel_ms, alive
+
+    setup_s = time.perf_counter() - t_setup
+    tracked, wall, kernel_ms, alive = run(args.mode, args.steps, args.warmup)
+    value = tracked / wall
+    kernel_s = kernel_ms / 1e3
+    achieved_tflops = tracked * flops_pt / kernel_s / 1e12
+    # HBM bytes the kernel must move: particle state read + written once per launch
+    alg_bytes = 2 * 168 * args.particles
+
+    extra = {}
+    if args.exact_steps > 0:
+        tr_e, wall_e, kms_e, _ = run("exact" if args.mode == "fma" else "fma",
+                                     args.exact_steps, 0)
+        extra = {"other_build": "exact" if args.mode == "fma" else "fma",
+                 "other_build_value": tr_e / wall_e,
+                 "other_build_tflops": tr_e * flops_pt / (kms_e / 1e3) / 1e12}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        v, threads, desc = cpu_baseline(eb, pb, budget_s=args.cpu_budget)
+        cpu = {"value": v, "unit": "particle-turns/s", "cores": threads, "kind": "port",
+               "sample": desc}
+
+    ms_per_step = wall * 1e3 / max(1, args.steps)
+    out = {
+        "metric": "FP64 particle-turns/s, 20k-elem ring, 1M particles; batched & adjoint vs plain",
+        "value": value,
+        "unit": "particle-turns/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "plain track_until, lhc_like_ring(seed=0): 20000 elements "
+                               "(10800 Drift, 7992 Multipole order 0-10, 400 XYShift, "
+                               "400 SRotation, 400 LimitEllipse, 8 Cavity); %d particles "
+                               "(polar grid 0-20 sigma x 64 angles); %d turns"
+                               % (args.particles, args.steps * tps),
+                   "build": args.mode, "turns_per_step": tps, "block": args.block,
+                   "particles": args.particles, "elements": eb.n_objects,
+                   "flops_per_particle_turn": flops_pt,
+                   "tracked_particle_turns": tracked, "alive_at_end": alive,
+                   "kernel_seconds": kernel_s, "setup_seconds": setup_s},
+        "roofline": {"bound": "fp64_valu", "achieved": achieved_tflops,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                     "algorithmic_hbm_bytes_per_launch": alg_bytes},
+        "cpu_baseline": cpu,
+    }
+    out.update(extra)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -137,6 +137,14 @@ ST_API NS(buffer_size_t) NS(TrackJob_get_num_beam_monitors)( const NS(TrackJobBa
 ST_API NS(Buffer)* NS(TrackJob_get_output_buffer)( NS(TrackJobBase)* job );
 ST_API NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* job );
 
+/* Timing and build-variant introspection (no reference counterpart):
+ * device time of the kernels of the last track_* call, measured with HIP
+ * events on the job's stream; whether the FMA build and the full-element
+ * kernel (beam-beam, space charge, monitors, RF multipoles) are used. */
+ST_API double NS(TrackJob_last_kernel_time_ms)( const NS(TrackJobBase)* job );
+ST_API int NS(TrackJob_uses_fma)( const NS(TrackJobBase)* job );
+ST_API int NS(TrackJob_uses_full_kernel)( const NS(TrackJobBase)* job );
+
 /* Output-buffer sizing used by the Python binding before it builds its
  * own output buffer (output/output_buffer.c:44-131 + 288-363):
  * reports the particle-set sizes the job will need, in the order the

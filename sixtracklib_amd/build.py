@@ -51,8 +51,12 @@ def _jobs():
             if not os.path.exists(os.path.join(CSRC, src)):
                 continue
             out = os.path.join(BUILD, src + variant + ".o")
-            j.append(([HIPCC] + HIP_COMMON + ["-ffp-contract=" + contract,
-                                              "-DST_KSUFFIX=" + variant, "-c",
+            defs = ["-DST_FMA_BUILD=1"] if variant == "_fma" else []
+            # the element dispatch branches only on wave-uniform values:
+            # leave those regions unstructurized (no flow blocks / copies)
+            defs += ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
+            j.append(([HIPCC] + HIP_COMMON + defs + ["-ffp-contract=" + contract,
+                                                     "-DST_KSUFFIX=" + variant, "-c",
                                               os.path.join(CSRC, src), "-o", out],
                       out))
     return j

@@ -49,16 +49,27 @@ struct LaneConst
     double q0, mass0, beta0, p0c, chi, charge_ratio;
 };
 
-// be_drift/track.h:45-67
+// be_drift/track.h:45-67.  The FMA build evaluates
+// rvv - (1 + (xp^2 + yp^2)/2) as (rvv - 1) - (xp^2 + yp^2)/2 with fused
+// multiply-adds (same quantity, different rounding).
 ST_DEV void drift( Lane& p, double const length )
 {
     double const xp = p.px * p.rpp;
     double const yp = p.py * p.rpp;
+#if defined( ST_FMA_BUILD )
+    double const t = fma( xp, xp, yp * yp );
+    double const dzeta = fma( t, -0.5, p.rvv - 1.0 );
+    p.x = fma( xp, length, p.x );
+    p.y = fma( yp, length, p.y );
+    p.s += length;
+    p.zeta = fma( length, dzeta, p.zeta );
+#else
     double const dzeta = p.rvv - ( 1.0 + ( xp * xp + yp * yp ) / 2.0 );
     p.x += xp * length;
     p.y += yp * length;
     p.s += length;
     p.zeta += length * dzeta;
+#endif
 }
 
 // be_drift/track.h:71-97
@@ -73,12 +84,69 @@ ST_DEV void drift_exact( Lane& p, double const length )
     p.zeta += dzeta;
 }
 
-// be_limit/track.h:54-88; sign(x)*x < 1 is |x| < 1 for every input
-// (+-0, +-Inf, NaN included)
-ST_DEV void limit_global( Lane& p )
+// be_limit/track.h:54-88: the particle survives iff sign(x)*x < 1 and
+// sign(y)*y < 1, which is |x| < 1 and |y| < 1 for every input (+-0, +-Inf
+// and NaN included).  Returns the survival predicate; the caller applies
+// state &= ok.
+ST_DEV bool limit_global_ok( const Lane& p )
 {
-    bool const ok = ( fabs( p.x ) < 1.0 ) & ( fabs( p.y ) < 1.0 );
-    p.state = ok ? p.state : 0;
+    return ( fabs( p.x ) < 1.0 ) & ( fabs( p.y ) < 1.0 );
+}
+
+// One Horner step of be_multipole/track.h:62-76 (dpx + i dpy) <- bal +
+// (dpx + i dpy)(x + i y).  The exact build keeps the reference's order of
+// operations; the FMA build folds each component into two fused
+// multiply-adds: fma(dpx, x, fma(-dpy, y, bal_re)).
+ST_DEV void horner_step( double& dpx, double& dpy, double const x,
+                         double const y, double const bre, double const bim )
+{
+#if defined( ST_FMA_BUILD )
+    double const ndpx = fma( dpx, x, fma( -dpy, y, bre ) );
+    dpy = fma( dpx, y, fma( dpy, x, bim ) );
+    dpx = ndpx;
+#else
+    double const zre = dpx * x - dpy * y;
+    double const zim = dpx * y + dpy * x;
+    dpx = bre + zre;
+    dpy = bim + zim;
+#endif
+}
+
+// Horner over a compile-time order: all 2N+2 coefficients are read with
+// constant offsets, so they arrive in one batch of scalar loads.
+template< int N >
+ST_DEV void horner_fixed( cptr_t e, double const x, double const y,
+                          double& dpx, double& dpy )
+{
+    dpx = rd( e, 2 * N );
+    dpy = rd( e, 2 * N + 1 );
+#pragma unroll
+    for( int k = N; k > 0; --k )
+        horner_step( dpx, dpy, x, y, rd( e, 2 * k - 2 ), rd( e, 2 * k - 1 ) );
+}
+
+ST_DEV void horner( cptr_t e, int const order, double const x, double const y,
+                    double& dpx, double& dpy )
+{
+    switch( order )
+    {
+    case 0: horner_fixed< 0 >( e, x, y, dpx, dpy ); return;
+    case 1: horner_fixed< 1 >( e, x, y, dpx, dpy ); return;
+    case 2: horner_fixed< 2 >( e, x, y, dpx, dpy ); return;
+    case 3: horner_fixed< 3 >( e, x, y, dpx, dpy ); return;
+    case 4: horner_fixed< 4 >( e, x, y, dpx, dpy ); return;
+    case 5: horner_fixed< 5 >( e, x, y, dpx, dpy ); return;
+    case 6: horner_fixed< 6 >( e, x, y, dpx, dpy ); return;
+    case 7: horner_fixed< 7 >( e, x, y, dpx, dpy ); return;
+    case 8: horner_fixed< 8 >( e, x, y, dpx, dpy ); return;
+    case 9: horner_fixed< 9 >( e, x, y, dpx, dpy ); return;
+    case 10: horner_fixed< 10 >( e, x, y, dpx, dpy ); return;
+    default: break;
+    }
+    int ix = 2 * order;
+    dpx = rd( e, ix );
+    dpy = rd( e, ix + 1 );
+    for( ; ix > 0; ix -= 2 ) horner_step( dpx, dpy, x, y, rd( e, ix - 2 ), rd( e, ix - 1 ) );
 }
 
 // be_multipole/track.h:38-112
@@ -86,17 +154,9 @@ ST_DEV void multipole( Lane& p, const LaneConst& c, cptr_t e, unsigned aux,
                        double const delta )
 {
     int const order = ( int )( aux & 0x7fffu );
-    int ix = 2 * order;
-    double dpx = rd( e, ix );
-    double dpy = rd( e, ix + 1 );
+    double dpx, dpy;
     double const x = p.x, y = p.y;
-    for( ; ix > 0 ; ix -= 2 )
-    {
-        double const zre = dpx * x - dpy * y;
-        double const zim = dpx * y + dpy * x;
-        dpx = rd( e, ix - 2 ) + zre;
-        dpy = rd( e, ix - 1 ) + zim;
-    }
+    horner( e, order, x, y, dpx, dpy );
     dpx = -c.chi * dpx;
     dpy = c.chi * dpy;
     if( aux & ST_MP_CURV_FLAG )
@@ -183,16 +243,17 @@ ST_DEV void srotation( Lane& p, cptr_t e )
     p.py = -sin_z * px + cos_z * py;
 }
 
+// Aperture predicates (the caller applies state &= ok,
+// particles.h:6493-6505).
 // be_limit/track.h:90-111
-ST_DEV void limit_rect( Lane& p, cptr_t e )
+ST_DEV bool limit_rect_ok( const Lane& p, cptr_t e )
 {
-    bool const ok = ( p.x >= rd( e, 0 ) ) && ( p.x <= rd( e, 1 ) ) &&
-                    ( p.y >= rd( e, 2 ) ) && ( p.y <= rd( e, 3 ) );
-    p.state &= ( int64_t )ok;
+    return ( p.x >= rd( e, 0 ) ) && ( p.x <= rd( e, 1 ) ) &&
+           ( p.y >= rd( e, 2 ) ) && ( p.y <= rd( e, 3 ) );
 }
 
 // be_limit/track.h:113-137
-ST_DEV void limit_ellipse( Lane& p, cptr_t e )
+ST_DEV bool limit_ellipse_ok( const Lane& p, cptr_t e )
 {
     double temp = p.x;
     double y_squ = p.y;
@@ -201,19 +262,18 @@ ST_DEV void limit_ellipse( Lane& p, cptr_t e )
     y_squ *= y_squ;
     y_squ *= rd( e, 0 );
     temp += y_squ;
-    p.state &= ( int64_t )( temp <= rd( e, 2 ) );
+    return temp <= rd( e, 2 );
 }
 
 // be_limit/track.h:139-162
-ST_DEV void limit_rect_ellipse( Lane& p, cptr_t e )
+ST_DEV bool limit_rect_ellipse_ok( const Lane& p, cptr_t e )
 {
     double const x = p.x, y = p.y;
     double const max_x = rd( e, 0 ), max_y = rd( e, 1 );
     double temp = x * x * rd( e, 3 );
     temp += y * y * rd( e, 2 );
-    bool const ok = ( x <= max_x ) && ( x >= -max_x ) && ( y <= max_y ) &&
-                    ( y >= -max_y ) && ( temp <= rd( e, 4 ) );
-    p.state &= ( int64_t )ok;
+    return ( x <= max_x ) && ( x >= -max_x ) && ( y <= max_y ) &&
+           ( y >= -max_y ) && ( temp <= rd( e, 4 ) );
 }
 
 // be_dipedge/track.h:37-51

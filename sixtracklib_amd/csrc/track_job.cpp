@@ -144,6 +144,8 @@ struct NS(TrackJobBase)
     std::vector< uint64_t > lat_host;
     bool lat_full = false;
     int32_t* d_status = nullptr; int32_t* h_status = nullptr;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    float last_kernel_ms = 0.0f;
     std::vector< st_dev_particles_t > views;
     st_dev_particles_t ebe_view{};
 
@@ -151,14 +153,16 @@ struct NS(TrackJobBase)
 
     void release()
     {
-        if( d_pbuf ) hipFree( d_pbuf );
-        if( d_obuf ) hipFree( d_obuf );
-        if( d_lat ) hipFree( d_lat );
-        if( d_status ) hipFree( d_status );
-        if( h_status ) hipHostFree( h_status );
-        if( stream ) hipStreamDestroy( stream );
+        if( d_pbuf ) ( void )hipFree( d_pbuf );
+        if( d_obuf ) ( void )hipFree( d_obuf );
+        if( d_lat ) ( void )hipFree( d_lat );
+        if( d_status ) ( void )hipFree( d_status );
+        if( h_status ) ( void )hipHostFree( h_status );
+        if( ev_start ) ( void )hipEventDestroy( ev_start );
+        if( ev_stop ) ( void )hipEventDestroy( ev_stop );
+        if( stream ) ( void )hipStreamDestroy( stream );
         d_pbuf = d_obuf = nullptr; d_lat = nullptr; d_status = nullptr;
-        h_status = nullptr; stream = nullptr;
+        h_status = nullptr; stream = nullptr; ev_start = ev_stop = nullptr;
         if( owns_obuf && obuf ) NS(Buffer_delete)( obuf );
         obuf = nullptr; owns_obuf = false;
     }
@@ -208,7 +212,8 @@ struct NS(TrackJobBase)
         const unsigned char* eb = NS(Buffer_get_const_data_begin)( ebuf );
         int64_t const nw = NS(Lattice_pack)( eb, nullptr, 0, nullptr, &n_elem_l, nullptr, &n_mon );
         if( nw < 0 ) { g_err = "lattice could not be packed"; return false; }
-        lat_host.assign( std::max< int64_t >( nw, 1 ), 0u );
+        // zero padding: the kernels read the header one record ahead
+        lat_host.assign( nw + 16, 0u );
         rec_offsets.assign( n_elem_l + 1, 0u );
         monitor_fields.assign( n_mon, 0u );
         if( NS(Lattice_pack)( eb, lat_host.data(), nw, rec_offsets.data(), &n_elem_l,
@@ -238,7 +243,7 @@ struct NS(TrackJobBase)
                 }
             }
         }
-        if( d_lat ) { hipFree( d_lat ); d_lat = nullptr; }
+        if( d_lat ) { ( void )hipFree( d_lat ); d_lat = nullptr; }
         if( !hip_ok( hipMalloc( &d_lat, lat_host.size() * 8u ), "hipMalloc lattice" ) ) return false;
         return hip_ok( hipMemcpyAsync( d_lat, lat_host.data(), lat_host.size() * 8u,
                                        hipMemcpyHostToDevice, stream ), "upload lattice" ) &&
@@ -319,6 +324,8 @@ struct NS(TrackJobBase)
 
         if( !hip_ok( hipSetDevice( device ), "hipSetDevice" ) ) return false;
         if( !hip_ok( hipStreamCreate( &stream ), "hipStreamCreate" ) ) return false;
+        if( !hip_ok( hipEventCreate( &ev_start ), "hipEventCreate" ) ||
+            !hip_ok( hipEventCreate( &ev_stop ), "hipEventCreate" ) ) return false;
         if( !hip_ok( hipMalloc( &d_status, sizeof( int32_t ) ), "hipMalloc status" ) ) return false;
         if( !hip_ok( hipHostMalloc( &h_status, sizeof( int32_t ) ), "hipHostMalloc" ) ) return false;
         pbuf_bytes = NS(Buffer_get_size)( pbuf );
@@ -337,18 +344,24 @@ struct NS(TrackJobBase)
         return true;
     }
 
+    // status word cleared before and read back after the launches; the two
+    // events bracket the kernels on the job stream (kernel time only)
     bool begin_status()
     {
-        return hip_ok( hipMemsetAsync( d_status, 0, sizeof( int32_t ), stream ), "hipMemsetAsync" );
+        return hip_ok( hipMemsetAsync( d_status, 0, sizeof( int32_t ), stream ), "hipMemsetAsync" ) &&
+               hip_ok( hipEventRecord( ev_start, stream ), "hipEventRecord" );
     }
 
     int32_t end_status( bool launched_ok )
     {
         if( !launched_ok ) { last_status = -1; return -1; }
-        if( !hip_ok( hipMemcpyAsync( h_status, d_status, sizeof( int32_t ),
+        if( !hip_ok( hipEventRecord( ev_stop, stream ), "hipEventRecord" ) ||
+            !hip_ok( hipMemcpyAsync( h_status, d_status, sizeof( int32_t ),
                                      hipMemcpyDeviceToHost, stream ), "status D2H" ) ||
             !hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" ) )
         { last_status = -1; return -1; }
+        float ms = 0.0f;
+        last_kernel_ms = ( hipEventElapsedTime( &ms, ev_start, ev_stop ) == hipSuccess ) ? ms : -1.0f;
         last_status = *h_status;
         return last_status;
     }
@@ -622,6 +635,9 @@ NS(buffer_size_t) NS(TrackJob_get_num_elem_by_elem_turns)( const NS(TrackJobBase
 NS(buffer_size_t) NS(TrackJob_get_num_beam_monitors)( const NS(TrackJobBase)* j ) { return j ? j->plan.monitor_elems.size() : 0u; }
 NS(Buffer)* NS(TrackJob_get_output_buffer)( NS(TrackJobBase)* j ) { return j ? j->obuf : nullptr; }
 NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* j ) { return j ? j->last_status : -1; }
+double NS(TrackJob_last_kernel_time_ms)( const NS(TrackJobBase)* j ) { return j ? ( double )j->last_kernel_ms : -1.0; }
+int NS(TrackJob_uses_fma)( const NS(TrackJobBase)* j ) { return ( j && j->fma ) ? 1 : 0; }
+int NS(TrackJob_uses_full_kernel)( const NS(TrackJobBase)* j ) { return ( j && j->lat_full ) ? 1 : 0; }
 
 int64_t NS(TrackJob_required_output_sets)( NS(Buffer)* pbuf, NS(buffer_size_t) nsets,
     NS(buffer_size_t) const* idx, NS(Buffer)* ebuf, NS(buffer_size_t) ute,

@@ -2,22 +2,28 @@
 //
 // This file is compiled twice (see sixtracklib_amd/build.py):
 //   ST_KSUFFIX=_exact  -ffp-contract=off : bit-identical to the CPU oracle
-//   ST_KSUFFIX=_fma    -ffp-contract=fast: fused multiply-adds (performance)
+//   ST_KSUFFIX=_fma    -ffp-contract=fast -DST_FMA_BUILD: fused multiply-adds
 //
 // Kernels replace the bodies of the reference's per-launch kernels
 // (track/track_kernel_impl.h:102-147 until-turn, 252-306 element by
 // element, 425-474 line; CUDA launch shape cuda/kernels/track_particles.cu)
-// and the per-particle loops (track/track.h:610-822).  Differences in
-// structure, not in results:
-//   * the lattice is the packed record stream of lattice.h read through a
-//     constant-address-space pointer -> scalar loads, scalar dispatch;
-//   * the 13 mutable particle fields stay in VGPRs for the whole launch;
-//   * at_element is not incremented per element: within a turn it is the
-//     start value plus the (wave-uniform) element counter, materialised at
-//     loss, at output stores and at the end;
-//   * a lost lane leaves the element loop (divergent break) and idles until
-//     the wave's other lanes finish the turn, exactly as the reference stops
-//     tracking it.
+// and the per-particle loops (track/track.h:610-822).
+//
+// Execution model (differs from the reference in structure, not results):
+//   * The lattice is the packed record stream of lattice.h, read through a
+//     constant-address-space pointer: every lane of a wave reads the same
+//     record, so the loads are scalar loads and the type dispatch is a
+//     scalar branch.  The next record's header is fetched one element
+//     ahead.
+//   * The 11 mutable particle fields stay in VGPRs for the whole launch.
+//   * The element loop is wave-uniform: a particle that is lost (or whose
+//     map fails) has its state written to memory at that element, exactly
+//     what the reference leaves behind, and is then masked out of every
+//     later store; the wave leaves the loop when no lane is active.  No
+//     lane ever branches out of the loop on its own, which keeps the
+//     per-element dispatch free of divergent control flow.
+//   * at_element is the lane's start value plus the wave-uniform element
+//     counter and is only materialised at losses, output stores and the end.
 #include <hip/hip_runtime.h>
 
 #include "track_device.h"
@@ -35,6 +41,13 @@ using namespace st;
 namespace
 {
 
+enum { MODE_UNTIL = 0, MODE_EBE = 1, MODE_LINE = 2 };
+
+// Lane bookkeeping lives in wave-uniform 64-bit masks (SGPRs): the vote of
+// a predicate and the test of this lane's bit in a mask.
+ST_DEV uint64_t ballot( bool p ) { return __builtin_amdgcn_ballot_w64( p ); }
+ST_DEV bool lane_in( uint64_t m ) { return ( ( m >> __lane_id() ) & 1u ) != 0u; }
+
 ST_DEV void load_lane( const st_dev_particles_t& P, int64_t i, Lane& L, LaneConst& C )
 {
     L.x = P.x[ i ]; L.px = P.px[ i ]; L.y = P.y[ i ]; L.py = P.py[ i ];
@@ -45,12 +58,15 @@ ST_DEV void load_lane( const st_dev_particles_t& P, int64_t i, Lane& L, LaneCons
     C.p0c = P.p0c[ i ]; C.chi = P.chi[ i ]; C.charge_ratio = P.charge_ratio[ i ];
 }
 
-ST_DEV void store_lane( const st_dev_particles_t& P, int64_t i, const Lane& L )
+ST_DEV void store_lane( const st_dev_particles_t& P, int64_t i, const Lane& L,
+                        int64_t at_element, int64_t at_turn )
 {
     P.x[ i ] = L.x; P.px[ i ] = L.px; P.y[ i ] = L.y; P.py[ i ] = L.py;
     P.zeta[ i ] = L.zeta; P.delta[ i ] = L.delta; P.rpp[ i ] = L.rpp;
     P.rvv[ i ] = L.rvv; P.psigma[ i ] = L.psigma; P.s[ i ] = L.s;
     P.state[ i ] = L.state;
+    P.at_element[ i ] = at_element;
+    P.at_turn[ i ] = at_turn;
 }
 
 // Full 21-field record (particles.h:1695-1803 field order) into output
@@ -69,7 +85,7 @@ ST_DEV void store_record( const st_dev_particles_t& O, int64_t k,
     O.state[ k ] = L.state;
 }
 
-// device copy of an output particle set header, addressed by the monitor
+// device copy of an output particle set header, addressed by a monitor
 ST_DEV st_dev_particles_t load_set( uint64_t addr )
 {
     cptr_t w = ( cptr_t )addr;
@@ -82,12 +98,14 @@ ST_DEV st_dev_particles_t load_set( uint64_t addr )
     return O;
 }
 
-// be_monitor/track.h:47-170
-ST_DEV_NOINLINE int monitor( cptr_t e, const st_dev_particles_t& P, int64_t i,
-    Lane& L, const LaneConst& C, int64_t pid, int64_t at_element, int64_t at_turn )
+// be_monitor/track.h:47-170.  Returns true when the copy fails (index past
+// the output set): the reference then marks the particle lost, status -1.
+ST_DEV_NOINLINE bool monitor( cptr_t e, const st_dev_particles_t& P, int64_t i,
+    const Lane& L, const LaneConst& C, int64_t pid, int64_t at_element,
+    int64_t at_turn, bool active )
 {
     uint64_t const out = e[ 3 ];
-    if( out == 0u ) return 0;
+    if( out == 0u ) return false;
     int64_t const num_stores = ri( e, 0 ), start = ri( e, 1 ), skip = ri( e, 2 );
     int64_t const max_pid = ri( e, 4 ), min_pid = ri( e, 5 );
     int64_t const is_rolling = ri( e, 6 ), is_turn_ordered = ri( e, 7 );
@@ -104,56 +122,53 @@ ST_DEV_NOINLINE int monitor( cptr_t e, const st_dev_particles_t& P, int64_t i,
             out_idx = is_turn_ordered ? nstore * store_idx + off : off * nstore + store_idx;
         }
     }
-    if( out_idx < 0 ) return 0;
+    if( out_idx < 0 ) return false;
     st_dev_particles_t const O = load_set( out );
-    if( out_idx >= O.n ) { L.state = 0; return -1; }   // failed copy
-    ( void )i;
-    store_record( O, out_idx, P, i, L, C, pid, at_element, at_turn );
-    return 0;
+    if( out_idx >= O.n ) return true;
+    if( active ) store_record( O, out_idx, P, i, L, C, pid, at_element, at_turn );
+    return false;
 }
 
-struct Where
-{
-    int64_t pid, at_element, at_turn;
-};
-
-// One element (track/track.h:209-585).  Returns the map status.
+// Elements other than Drift and Multipole (track/track.h:209-585).
+// ``ok`` = false: the particle is lost at this element (state &= 0);
+// ``fail`` = true: the map returned a failure status (-1).
 template< bool FULL >
-ST_DEV int apply( uint64_t h, cptr_t e, Lane& L, const LaneConst& C,
-                  const st_dev_particles_t& P, int64_t i, const Where& w )
+ST_DEV void step_other( unsigned op, unsigned aux, cptr_t e, Lane& L, const LaneConst& C,
+                        bool& ok, bool& fail, bool active, const st_dev_particles_t& P,
+                        int64_t i, int64_t pid, int64_t at_element, int64_t at_turn )
 {
-    unsigned const op = ST_REC_OP( h );
-    if( op == ST_OP_DRIFT ) { drift( L, rd( e, 0 ) ); limit_global( L ); return 0; }
-    if( op == ST_OP_MULTIPOLE ) { multipole( L, C, e, ST_REC_AUX( h ), L.delta ); return 0; }
     switch( op )
     {
-    case ST_OP_DRIFT_EXACT: drift_exact( L, rd( e, 0 ) ); limit_global( L ); return 0;
-    case ST_OP_CAVITY: cavity( L, C, e ); return 0;
-    case ST_OP_XYSHIFT: xyshift( L, e ); return 0;
-    case ST_OP_SROTATION: srotation( L, e ); return 0;
-    case ST_OP_LIMIT_RECT: limit_rect( L, e ); return 0;
-    case ST_OP_LIMIT_ELLIPSE: limit_ellipse( L, e ); return 0;
-    case ST_OP_LIMIT_RECT_ELLIPSE: limit_rect_ellipse( L, e ); return 0;
-    case ST_OP_DIPEDGE: dipedge( L, e ); return 0;
+    case ST_OP_DRIFT_EXACT: drift_exact( L, rd( e, 0 ) ); ok = limit_global_ok( L ); return;
+    case ST_OP_CAVITY: cavity( L, C, e ); return;
+    case ST_OP_XYSHIFT: xyshift( L, e ); return;
+    case ST_OP_SROTATION: srotation( L, e ); return;
+    case ST_OP_LIMIT_RECT: ok = limit_rect_ok( L, e ); return;
+    case ST_OP_LIMIT_ELLIPSE: ok = limit_ellipse_ok( L, e ); return;
+    case ST_OP_LIMIT_RECT_ELLIPSE: ok = limit_rect_ellipse_ok( L, e ); return;
+    case ST_OP_DIPEDGE: dipedge( L, e ); return;
     default: break;
     }
     if( FULL )
     {
         switch( op )
         {
-        case ST_OP_MONITOR: return monitor( e, P, i, L, C, w.pid, w.at_element, w.at_turn );
-        case ST_OP_BB4D: bb4d( L, C, e ); return 0;
-        case ST_OP_BB6D: bb6d( L, C, e ); return 0;
-        case ST_OP_SC_COASTING: sc_coasting( L, C, e ); return 0;
-        case ST_OP_SC_QGAUSS: sc_qgauss( L, C, e ); return 0;
-        case ST_OP_SC_INTERPOL: sc_interpol( L, C, e ); return 0;
-        case ST_OP_RF_MULTIPOLE: rf_multipole( L, C, e, ST_REC_AUX( h ) ); return 0;
-        case ST_OP_FAIL: return -1;
+        case ST_OP_MONITOR:
+            if( monitor( e, P, i, L, C, pid, at_element, at_turn, active ) )
+            { ok = false; fail = true; }
+            return;
+        case ST_OP_BB4D: bb4d( L, C, e ); return;
+        case ST_OP_BB6D: bb6d( L, C, e ); return;
+        case ST_OP_SC_COASTING: sc_coasting( L, C, e ); return;
+        case ST_OP_SC_QGAUSS: sc_qgauss( L, C, e ); return;
+        case ST_OP_SC_INTERPOL: sc_interpol( L, C, e ); return;
+        case ST_OP_RF_MULTIPOLE: rf_multipole( L, C, e, aux ); return;
+        case ST_OP_FAIL: fail = true; return;           // disabled map
         default: break;
         }
     }
-    L.state = 0;                            // unknown type: track.h:577-581
-    return -1;
+    ok = false;                                           // unknown type:
+    fail = true;                                          // track.h:577-581
 }
 
 // output/elem_by_elem_config.h:663-760
@@ -177,178 +192,133 @@ ST_DEV int64_t ebe_index( const st_ebe_config_t& c, int64_t pid, int64_t at_elem
     return out;
 }
 
-ST_DEV void report( int32_t* status_out, int32_t st )
+struct Args
 {
-    if( st != 0 ) atomicOr( status_out, st );
-}
+    cptr_t lat;
+    int64_t n_elem;          // elements per turn (until / ebe) or in the line
+    int64_t until_turn;
+    bool finish_turn;        // line mode
+    st_ebe_config_t cfg;     // ebe mode
+    st_dev_particles_t O;    // ebe output set
+    int32_t* status_out;
+};
 
-// track/track.h:610-670 + track_kernel_impl.h:102-147
-template< bool FULL >
-ST_DEV void until_body( const st_dev_particles_t& P, int64_t i, cptr_t lat,
-                        int64_t n_elem, int64_t until_turn, int32_t* status_out )
+// Per-particle loops: track/track.h:610-670 (until), 672-777 (element by
+// element), 779-822 (line), with the kernel bodies of
+// track_kernel_impl.h:102-147, 252-306, 425-474.
+template< int MODE, bool FULL >
+ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
 {
+    bool const in_range = i < P.n;
+    int64_t const li = in_range ? i : 0;          // out-of-range lanes never store
     Lane L; LaneConst C;
-    load_lane( P, i, L, C );
-    int64_t const start_elem = P.at_element[ i ];
-    int64_t at_turn = P.at_turn[ i ];
-    int64_t at_element = start_elem;
-    Where w = { FULL ? P.particle_id[ i ] : 0, start_elem, at_turn };
-    int32_t status = ( n_elem > 0 ) ? 0 : -1;
-    bool go = ( n_elem > 0 ) && ( until_turn > at_turn ) && ( L.state == 1 );
-    bool const tracked = go;
-    while( go )
-    {
-        cptr_t r = lat;
-        int64_t e = 0;
-        int st = 0;
-        for( ; e < n_elem; ++e )
-        {
-            uint64_t const h = r[ 0 ];
-            if( FULL ) { w.at_element = start_elem + e; w.at_turn = at_turn; }
-            st = apply< FULL >( h, r + 1, L, C, P, i, w );
-            r += ST_REC_LEN( h );
-            if( st != 0 || L.state != 1 ) break;
-        }
-        status = st;
-        if( e == n_elem )
-        {
-            ++at_turn;
-            L.s = 0.0;
-            go = at_turn < until_turn;
-        }
-        else
-        {
-            at_element = start_elem + e;
-            L.state = 0;
-            go = false;
-        }
-    }
-    if( tracked )
-    {
-        store_lane( P, i, L );
-        P.at_element[ i ] = at_element;
-        P.at_turn[ i ] = at_turn;
-    }
-    report( status_out, status );
-}
+    load_lane( P, li, L, C );
+    int64_t const start = P.at_element[ li ];
+    int64_t at_turn = P.at_turn[ li ];
+    int64_t const pid = ( FULL || MODE == MODE_EBE ) ? P.particle_id[ li ] : 0;
+    int const n_elem = ( int )a.n_elem;           // host guarantees < 2^31
 
-// track/track.h:672-777 + track_kernel_impl.h:252-306
-ST_DEV void ebe_body( const st_dev_particles_t& P, int64_t i, cptr_t lat,
-                      int64_t n_elem, int64_t until_turn, const st_ebe_config_t& cfg,
-                      const st_dev_particles_t& O, int32_t* status_out )
-{
-    Lane L; LaneConst C;
-    load_lane( P, i, L, C );
-    int64_t const pid = P.particle_id[ i ];
-    int64_t const start_elem = P.at_element[ i ];
-    int64_t at_turn = P.at_turn[ i ];
-    int64_t at_element = start_elem;
+    bool active0;
     int32_t status = 0;
-    bool go = ( n_elem > 0 ) && ( until_turn > at_turn ) && ( at_turn >= 0 ) &&
-              ( start_elem >= 0 ) && ( pid >= 0 );
-    bool const tracked = go;
-    while( go )
+    if( MODE == MODE_EBE )
+        active0 = in_range && n_elem > 0 && a.until_turn > at_turn && at_turn >= 0 &&
+                  start >= 0 && pid >= 0;
+    else if( MODE == MODE_LINE )
+        active0 = in_range && n_elem > 0 && L.state == 1;
+    else
     {
-        cptr_t r = lat;
-        int64_t e = 0;
-        int st = 0;
-        for( ; e < n_elem; ++e )
-        {
-            uint64_t const h = r[ 0 ];
-            int64_t const k = ebe_index( cfg, pid, start_elem + e, at_turn );
-            if( k >= 0 && k < O.n )
-                store_record( O, k, P, i, L, C, pid, start_elem + e, at_turn );
-            Where const w = { pid, start_elem + e, at_turn };
-            st = apply< true >( h, r + 1, L, C, P, i, w );
-            r += ST_REC_LEN( h );
-            if( st != 0 || L.state != 1 ) break;
-        }
-        status = st;
-        if( e == n_elem )
-        {
-            ++at_turn;
-            L.s = 0.0;
-            go = at_turn < until_turn;
-        }
-        else
-        {
-            at_element = start_elem + e;
-            go = false;
-        }
+        active0 = in_range && n_elem > 0 && a.until_turn > at_turn && L.state == 1;
+        if( n_elem <= 0 ) status = -1;            // empty lattice: failure
     }
-    if( tracked )
-    {
-        store_lane( P, i, L );
-        P.at_element[ i ] = at_element;
-        P.at_turn[ i ] = at_turn;
-    }
-    report( status_out, status );
-}
 
-// track/track.h:779-822 + track_kernel_impl.h:425-474
-ST_DEV void line_body( const st_dev_particles_t& P, int64_t i, cptr_t lat,
-                       int64_t n_line, bool finish_turn, int32_t* status_out )
-{
-    Lane L; LaneConst C;
-    load_lane( P, i, L, C );
-    int64_t const start_elem = P.at_element[ i ];
-    int64_t at_turn = P.at_turn[ i ];
-    int64_t at_element = start_elem;
-    Where w = { P.particle_id[ i ], start_elem, at_turn };
-    int st = 0;
-    bool const go = ( n_line > 0 ) && ( L.state == 1 );
-    if( go )
+    // lanes still being tracked; a lane leaves the mask exactly once, after
+    // its final state has been stored
+    uint64_t act = ballot( active0 );
+    while( act != 0u )
     {
-        cptr_t r = lat;
-        int64_t e = 0;
-        for( ; e < n_line; ++e )
+        cptr_t r = a.lat;
+        uint64_t h = r[ 0 ];
+        int e_end = n_elem;
+        for( int e = 0; e < e_end; ++e )
         {
-            uint64_t const h = r[ 0 ];
-            w.at_element = start_elem + e;
-            st = apply< true >( h, r + 1, L, C, P, i, w );
-            r += ST_REC_LEN( h );
-            if( st != 0 || L.state != 1 ) break;
+            cptr_t const nr = r + ST_REC_LEN( h );
+            uint64_t const hn = nr[ 0 ];          // next header (stream is padded)
+            unsigned const op = ST_REC_OP( h );
+            cptr_t const prm = r + 1;
+            if( MODE == MODE_EBE )
+            {
+                int64_t const k = ebe_index( a.cfg, pid, start + e, at_turn );
+                if( lane_in( act ) && k >= 0 && k < a.O.n )
+                    store_record( a.O, k, P, i, L, C, pid, start + e, at_turn );
+            }
+            uint64_t bad = 0u, failm = 0u;        // lost here / map failed
+            if( op == ST_OP_DRIFT )
+            {
+                drift( L, rd( prm, 0 ) );
+                bad = ballot( !limit_global_ok( L ) );
+            }
+            else if( op == ST_OP_MULTIPOLE )
+            {
+                multipole( L, C, prm, ST_REC_AUX( h ), L.delta );
+            }
+            else
+            {
+                bool ok = true, fail = false;
+                step_other< FULL >( op, ST_REC_AUX( h ), prm, L, C, ok, fail,
+                                    lane_in( act ), P, i, pid, start + e, at_turn );
+                bad = ballot( !ok );
+                failm = ballot( fail );
+            }
+            if( MODE == MODE_EBE ) bad |= ballot( L.state != 1 );
+            uint64_t const stop = ( bad | failm ) & act;
+            if( stop != 0u )
+            {
+                if( lane_in( stop ) )
+                {
+                    if( lane_in( bad ) || MODE == MODE_UNTIL ) L.state = 0;
+                    store_lane( P, i, L, start + e, at_turn );
+                    status = lane_in( failm ) ? -1 : 0;
+                }
+                act &= ~stop;
+                if( act == 0u ) e_end = 0;
+            }
+            r = nr;
+            h = hn;
         }
-        at_element = start_elem + e;
+        if( MODE == MODE_LINE ) break;
+        if( lane_in( act ) ) { ++at_turn; L.s = 0.0; }
+        uint64_t const fin = ballot( at_turn >= a.until_turn ) & act;
+        if( fin != 0u )
+        {
+            if( lane_in( fin ) ) store_lane( P, i, L, start, at_turn );
+            act &= ~fin;
+        }
     }
-    if( finish_turn && st == 0 && L.state == 1 )
+    if( MODE == MODE_LINE && lane_in( act ) )
     {
-        at_element = 0;
-        ++at_turn;
-        L.s = 0.0;
+        int64_t at_element = start + n_elem;
+        if( a.finish_turn ) { at_element = 0; ++at_turn; L.s = 0.0; }
+        store_lane( P, i, L, at_element, at_turn );
     }
-    store_lane( P, i, L );
-    P.at_element[ i ] = at_element;
-    P.at_turn[ i ] = at_turn;
-    report( status_out, st );
+    if( in_range && status != 0 ) atomicOr( a.status_out, status );
 }
 
 }  // namespace
 
 template< bool FULL >
-__global__ void __launch_bounds__( 256 ) KN( st_k_track_until )(
-    st_dev_particles_t P, const uint64_t* __restrict__ lat, int64_t n_elem,
-    int64_t until_turn, int32_t* status_out )
+__global__ void __launch_bounds__( 256 ) KN( st_k_track_until )( st_dev_particles_t P, Args a )
 {
-    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
-    if( i < P.n ) until_body< FULL >( P, i, ( cptr_t )lat, n_elem, until_turn, status_out );
+    track_body< MODE_UNTIL, FULL >( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
 }
 
-__global__ void __launch_bounds__( 256 ) KN( st_k_track_ebe )(
-    st_dev_particles_t P, const uint64_t* __restrict__ lat, int64_t n_elem,
-    int64_t until_turn, st_ebe_config_t cfg, st_dev_particles_t O,
-    int32_t* status_out )
+__global__ void __launch_bounds__( 256 ) KN( st_k_track_ebe )( st_dev_particles_t P, Args a )
 {
-    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
-    if( i < P.n ) ebe_body( P, i, ( cptr_t )lat, n_elem, until_turn, cfg, O, status_out );
+    track_body< MODE_EBE, true >( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
 }
 
-__global__ void __launch_bounds__( 256 ) KN( st_k_track_line )(
-    st_dev_particles_t P, const uint64_t* __restrict__ lat, int64_t n_line,
-    int finish_turn, int32_t* status_out )
+__global__ void __launch_bounds__( 256 ) KN( st_k_track_line )( st_dev_particles_t P, Args a )
 {
-    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
-    if( i < P.n ) line_body( P, i, ( cptr_t )lat, n_line, finish_turn != 0, status_out );
+    track_body< MODE_LINE, true >( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
 }
 
 // ------------------------------------------------------------ launchers
@@ -358,18 +328,26 @@ static inline unsigned grid_for( int64_t n, int block )
     return ( unsigned )( ( n + block - 1 ) / block );
 }
 
+static inline int sane_block( int block )
+{
+    return ( block <= 0 || block > 256 || ( block % 64 ) != 0 ) ? 256 : block;
+}
+
 extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
     const uint64_t* lat, int64_t n_elem, int64_t until_turn, int full,
     int32_t* status, int block, hipStream_t stream )
 {
     if( P->n <= 0 ) return hipSuccess;
-    if( block <= 0 || block > 256 || ( block % 64 ) != 0 ) block = 256;
+    block = sane_block( block );
+    Args a{};
+    a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
+    a.status_out = status;
     if( full )
         hipLaunchKernelGGL( KN( st_k_track_until )< true >, dim3( grid_for( P->n, block ) ),
-                            dim3( block ), 0, stream, *P, lat, n_elem, until_turn, status );
+                            dim3( block ), 0, stream, *P, a );
     else
         hipLaunchKernelGGL( KN( st_k_track_until )< false >, dim3( grid_for( P->n, block ) ),
-                            dim3( block ), 0, stream, *P, lat, n_elem, until_turn, status );
+                            dim3( block ), 0, stream, *P, a );
     return hipGetLastError();
 }
 
@@ -379,9 +357,12 @@ extern "C" hipError_t KN( st_launch_track_ebe )( const st_dev_particles_t* P,
     int block, hipStream_t stream )
 {
     if( P->n <= 0 ) return hipSuccess;
-    if( block <= 0 || block > 256 || ( block % 64 ) != 0 ) block = 256;
+    block = sane_block( block );
+    Args a{};
+    a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
+    a.cfg = *cfg; a.O = *O; a.status_out = status;
     hipLaunchKernelGGL( KN( st_k_track_ebe ), dim3( grid_for( P->n, block ) ), dim3( block ),
-                        0, stream, *P, lat, n_elem, until_turn, *cfg, *O, status );
+                        0, stream, *P, a );
     return hipGetLastError();
 }
 
@@ -390,8 +371,11 @@ extern "C" hipError_t KN( st_launch_track_line )( const st_dev_particles_t* P,
     int block, hipStream_t stream )
 {
     if( P->n <= 0 ) return hipSuccess;
-    if( block <= 0 || block > 256 || ( block % 64 ) != 0 ) block = 256;
+    block = sane_block( block );
+    Args a{};
+    a.lat = ( cptr_t )lat; a.n_elem = n_line; a.finish_turn = finish_turn != 0;
+    a.status_out = status;
     hipLaunchKernelGGL( KN( st_k_track_line ), dim3( grid_for( P->n, block ) ), dim3( block ),
-                        0, stream, *P, lat, n_line, finish_turn, status );
+                        0, stream, *P, a );
     return hipGetLastError();
 }

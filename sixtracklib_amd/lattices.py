@@ -112,95 +112,92 @@ def fodo_ring(n_cells=100, k1=0.01, k2=0.05, drift=1.0, voltage=12e6,
     return eb
 
 
-def lhc_like_ring(seed=0, n_cells=400, cell_len=50, mu_deg=72.0, rel_err=1e-4,
-                  k2_scale=0.02, r0=0.017, n_cavities=8, voltage=2e6,
-                  frequency=400e6, ellipse_a=0.02, ellipse_b=0.015,
+def lhc_like_ring(seed=0, n_cells=400, tune=80.31, rel_err=1e-4, k2_scale=0.02,
+                  r0=0.017, n_cavities=8, voltage=2e6, frequency=400e6,
+                  ellipse_a=0.02, ellipse_b=0.015, mean_drift=5.05,
                   error_seed=None, cbuffer=None, return_info=False):
-    """20,000-element synthetic ring (see module docstring).  ``seed`` fixes
-    the structure (drift lengths, element orders, placements);
-    ``error_seed`` (default ``seed``) the multipole error table, so machine
-    seeds of one ring differ only in their errors."""
+    """20,000-element synthetic ring (module docstring).
+
+    Each 50-element cell is [QF | 13 drifts, 9 multipoles, 1 shift/rotation
+    (shuffled) | QD | 14 drifts, 9 multipoles, 1 shift/rotation (shuffled) |
+    LimitEllipse].  Drift lengths are drawn U(0.1, 10) m and rescaled so
+    every half cell has the same total length (13 and 14 x ``mean_drift``):
+    the linear optics of all cells are then identical, which keeps the ring
+    stable, while the element sequence stays random.  The thin quads give a
+    thin-lens FODO phase advance mu with cos(mu) = 1 - l1 l2 k^2 / 2 and
+    ``tune`` = n_cells mu / 2pi (non-integer).  Non-quad multipoles have a
+    random order 0..10, half of those of order >= 2 carry a design
+    sextupole +-``k2_scale``; every multipole gets N(0, 1e-4) relative
+    errors on all its coefficients (normalised to the quad strength at the
+    reference radius ``r0``).  ``seed`` fixes the structure, ``error_seed``
+    (default ``seed``) the error table.  8 cells carry a cavity in place of
+    one multipole."""
     if error_seed is None:
         error_seed = seed
     rng = np.random.default_rng(seed)
     erng = np.random.default_rng(10_000 + int(error_seed))
+    cell_len = 50
     n_elem = n_cells * cell_len
     eb = cbuffer if cbuffer is not None else CBuffer(
         max_slots=n_elem * 16, max_objects=n_elem + 16, max_pointers=n_elem)
     cav_cells = set(np.linspace(0, n_cells - 1, n_cavities).astype(int).tolist())
-    n_drift, n_mp, n_misc = 27, 20, 2
+    l1, l2 = 13 * mean_drift, 14 * mean_drift
+    mu = 2.0 * math.pi * tune / n_cells
+    k1 = 2.0 * math.sin(mu / 2.0) / math.sqrt(l1 * l2)
     counts = {"drift": 0, "multipole": 0, "xyshift": 0, "srotation": 0,
               "ellipse": 0, "cavity": 0}
     orders = []
-    sin_half = math.sin(math.radians(mu_deg) / 2.0)
+    misc_toggle = 0
     for c in range(n_cells):
-        lengths = rng.uniform(0.1, 10.0, n_drift)
-        # slot layout: QF, then half of the other slots, QD, the rest; the
-        # ellipse is the cell's last element (every cell_len-th element)
-        others = ["drift"] * n_drift + ["mp"] * (n_mp - 2) + ["misc"] * n_misc
-        rng.shuffle(others)
-        half = len(others) // 2
-        seq = ["QF"] + others[:half] + ["QD"] + others[half:] + ["ellipse"]
-        assert len(seq) == cell_len
-        # focal length from the cell's own half-cell lengths (thin FODO)
-        di = iter(range(n_drift))
-        dl = [lengths[next(di)] if s == "drift" else 0.0 for s in seq]
-        l1 = sum(dl[: seq.index("QD")])
-        l2 = sum(dl[seq.index("QD"):])
-        lh = 0.5 * (l1 + l2)
-        k1 = 2.0 * sin_half / lh
-        cav_slot = None
-        if c in cav_cells:
-            cav_slot = [i for i, s in enumerate(seq) if s == "drift"][0]
-        di = iter(range(n_drift))
-        misc_toggle = 0
-        for i, s in enumerate(seq):
-            if s == "drift":
-                L = lengths[next(di)]
-                if i == cav_slot:
-                    be.Cavity(cbuffer=eb, voltage=voltage, frequency=frequency,
-                              lag=180.0)
-                    counts["cavity"] += 1
-                else:
-                    be.Drift(cbuffer=eb, length=L)
+        halves = []
+        for nd, total in ((13, l1), (14, l2)):
+            lengths = rng.uniform(0.1, 10.0, nd)
+            lengths *= total / lengths.sum()
+            slots = ["drift"] * nd + ["mp"] * 9 + ["misc"]
+            rng.shuffle(slots)
+            halves.append((slots, iter(lengths)))
+        cav_done = c not in cav_cells
+        for part, (slots, lengths) in zip(("QF", "QD"), halves):
+            for s in [part] + slots:
+                if s == "drift":
+                    be.Drift(cbuffer=eb, length=float(next(lengths)))
                     counts["drift"] += 1
-            elif s in ("QF", "QD", "mp"):
-                if s == "mp":
-                    order = int(rng.integers(0, 11))
-                else:
-                    order = int(rng.integers(1, 11))
-                knl = np.zeros(order + 1)
-                ksl = np.zeros(order + 1)
-                kref = k1
-                if s == "QF":
-                    knl[1] = +k1
-                elif s == "QD":
-                    knl[1] = -k1
-                elif order >= 2 and rng.random() < 0.5:
-                    knl[2] = k2_scale * (1.0 if rng.random() < 0.5 else -1.0)
-                # relative errors: b_n / b_ref at the reference radius r0
-                nn = np.arange(order + 1)
-                scale = rel_err * kref * np.array([math.factorial(int(m)) for m in nn]) \
-                    * r0 ** (1.0 - nn)
-                knl = knl + scale * erng.normal(0.0, 1.0, order + 1)
-                ksl = ksl + scale * erng.normal(0.0, 1.0, order + 1)
-                be.Multipole(cbuffer=eb, knl=knl, ksl=ksl)
-                orders.append(order)
-                counts["multipole"] += 1
-            elif s == "misc":
-                if misc_toggle == 0:
-                    be.XYShift(cbuffer=eb, dx=rng.normal(0, 1e-4), dy=rng.normal(0, 1e-4))
-                    counts["xyshift"] += 1
-                else:
-                    ang = rng.normal(0, 1e-4)
-                    be.SRotation(cbuffer=eb, cos_z=math.cos(ang), sin_z=math.sin(ang))
-                    counts["srotation"] += 1
-                misc_toggle ^= 1
-            elif s == "ellipse":
-                be.LimitEllipse(cbuffer=eb, a=ellipse_a, b=ellipse_b)
-                counts["ellipse"] += 1
+                elif s == "mp" and not cav_done:
+                    be.Cavity(cbuffer=eb, voltage=voltage, frequency=frequency, lag=180.0)
+                    counts["cavity"] += 1
+                    cav_done = True
+                elif s in ("QF", "QD", "mp"):
+                    order = int(rng.integers(0, 11)) if s == "mp" else int(rng.integers(1, 11))
+                    knl = np.zeros(order + 1)
+                    ksl = np.zeros(order + 1)
+                    if s == "QF":
+                        knl[1] = +k1
+                    elif s == "QD":
+                        knl[1] = -k1
+                    elif order >= 2 and rng.random() < 0.5:
+                        knl[2] = k2_scale * (1.0 if rng.random() < 0.5 else -1.0)
+                    nn = np.arange(order + 1)
+                    scale = rel_err * k1 * np.array([math.factorial(int(m)) for m in nn]) \
+                        * r0 ** (1.0 - nn)
+                    knl = knl + scale * erng.normal(0.0, 1.0, order + 1)
+                    ksl = ksl + scale * erng.normal(0.0, 1.0, order + 1)
+                    be.Multipole(cbuffer=eb, knl=knl, ksl=ksl)
+                    orders.append(order)
+                    counts["multipole"] += 1
+                elif s == "misc":
+                    if misc_toggle == 0:
+                        be.XYShift(cbuffer=eb, dx=rng.normal(0, 1e-4), dy=rng.normal(0, 1e-4))
+                        counts["xyshift"] += 1
+                    else:
+                        ang = rng.normal(0, 1e-4)
+                        be.SRotation(cbuffer=eb, cos_z=math.cos(ang), sin_z=math.sin(ang))
+                        counts["srotation"] += 1
+                    misc_toggle ^= 1
+        be.LimitEllipse(cbuffer=eb, a=ellipse_a, b=ellipse_b)
+        counts["ellipse"] += 1
     if return_info:
-        return eb, {"counts": counts, "orders": orders}
+        return eb, {"counts": counts, "orders": orders, "k1": k1, "mu": mu,
+                    "half_cells": (l1, l2)}
     return eb
 
 

@@ -118,6 +118,9 @@ _fn("st_TrackJob_last_status", st_track_status_t, [st_TrackJob_p])
 _fn("st_TrackJob_required_output_sets", ct.c_int64,
     [st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p, st_buffer_size_t,
      ct.POINTER(ct.c_int64), ct.c_int64, ct.POINTER(ct.c_int64)])
+_fn("st_TrackJob_last_kernel_time_ms", ct.c_double, [st_TrackJob_p])
+_fn("st_TrackJob_uses_fma", ct.c_int, [st_TrackJob_p])
+_fn("st_TrackJob_uses_full_kernel", ct.c_int, [st_TrackJob_p])
 _fn("st_Hip_num_devices", ct.c_int, [])
 _fn("st_Hip_last_error_str", ct.c_char_p, [])
 

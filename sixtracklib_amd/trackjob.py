@@ -182,6 +182,16 @@ class TrackJob:
     def requires_collecting(self):
         return True
 
+    @property
+    def last_kernel_time_ms(self):
+        """device time of the kernels of the last track_* call (HIP events
+        on the job's stream)"""
+        return st.st_TrackJob_last_kernel_time_ms(self.ptr_st_track_job)
+
+    @property
+    def uses_fma(self):
+        return bool(st.st_TrackJob_uses_fma(self.ptr_st_track_job))
+
     # -- tracking ----------------------------------------------------------
     def _check(self, status, what):
         self._last_track_status = status
