@@ -17,8 +17,10 @@ extern "C" {
 /* ``belems``: a beam-element CObjects image relocated to its own address.
  * Returns the number of 8-byte words of the packed stream, or -1 on error
  * (unrelocated buffer, element data outside the buffer, ...).  With
- * ``out`` == NULL only sizes are reported.  ``rec_offsets`` (optional,
- * n_elem + 1 entries) receives the word offset of every element record;
+ * ``out`` == NULL only sizes are reported.  The stream ends with an end
+ * record (2 words).  ``rec_offsets`` (optional, n_elem + 1 entries)
+ * receives the word offset of every element record and, last, of the end
+ * record;
  * ``monitor_fields`` (optional, one per BeamMonitor) the word index of the
  * monitor's output-address slot. */
 ST_API int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,

@@ -42,9 +42,12 @@ struct Packer
     bool inside( uint64_t addr, uint64_t bytes ) const
     { return addr >= lo && addr + bytes <= hi && ( addr % 8u ) == 0u; }
 
+    uint64_t index = 0;                       // position of the object being packed
+
     void rec( unsigned op, unsigned aux, const std::vector< uint64_t >& payload )
     {
-        w.push_back( ST_MAKE_HDR( op, aux, payload.size() + 1u ) );
+        w.push_back( ST_MAKE_HDR( op, aux, payload.size() + ST_REC_PAYLOAD ) );
+        w.push_back( index );
         w.insert( w.end(), payload.begin(), payload.end() );
     }
 
@@ -111,7 +114,7 @@ struct Packer
         {
             // out address is patched with the device address of the output
             // particle set by the job (0 = monitor inactive)
-            monitor_fields.push_back( static_cast< uint32_t >( w.size() + 4u ) );
+            monitor_fields.push_back( static_cast< uint32_t >( w.size() + ST_REC_PAYLOAD + 3u ) );
             rec( ST_OP_MONITOR, 0, { e[ 0 ], e[ 1 ], e[ 2 ], 0u, e[ 4 ], e[ 5 ], e[ 6 ], e[ 7 ] } );
             break;
         }
@@ -197,9 +200,13 @@ extern "C" int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,
     for( int64_t i = 0; i < n; ++i )
     {
         pk.offs.push_back( static_cast< uint32_t >( pk.w.size() ) );
+        pk.index = static_cast< uint64_t >( i );
         if( !pk.add( obj[ i ] ) ) return -1;
+        if( pk.w.size() >= ( 1u << 28 ) ) return -1;    // offsets and byte lengths fit 32 bits
     }
     pk.offs.push_back( static_cast< uint32_t >( pk.w.size() ) );
+    pk.index = static_cast< uint64_t >( n );
+    pk.rec( ST_OP_END, 0, {} );
     int64_t const nw = static_cast< int64_t >( pk.w.size() );
     if( n_elem_out ) *n_elem_out = n;
     if( n_monitors_out ) *n_monitors_out = static_cast< int64_t >( pk.monitor_fields.size() );

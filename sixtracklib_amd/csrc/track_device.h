@@ -72,6 +72,22 @@ ST_DEV void drift( Lane& p, double const length )
 #endif
 }
 
+#if defined( ST_FMA_BUILD )
+// The same map with rvv - 1 supplied by the caller (it changes only where
+// the energy does).
+ST_DEV void drift_rvvm1( Lane& p, double const length, double const rvvm1 )
+{
+    double const xp = p.px * p.rpp;
+    double const yp = p.py * p.rpp;
+    double const t = fma( xp, xp, yp * yp );
+    double const dzeta = fma( t, -0.5, rvvm1 );
+    p.x = fma( xp, length, p.x );
+    p.y = fma( yp, length, p.y );
+    p.s += length;
+    p.zeta = fma( length, dzeta, p.zeta );
+}
+#endif
+
 // be_drift/track.h:71-97
 ST_DEV void drift_exact( Lane& p, double const length )
 {
@@ -93,16 +109,52 @@ ST_DEV bool limit_global_ok( const Lane& p )
     return ( fabs( p.x ) < 1.0 ) & ( fabs( p.y ) < 1.0 );
 }
 
+// The same test as a wave mask of the lanes that fail it: one compare per
+// coordinate, each written straight into an SGPR pair.
+ST_DEV uint64_t limit_global_lost_mask( const Lane& p )
+{
+    return __builtin_amdgcn_ballot_w64( !( fabs( p.x ) < 1.0 ) ) |
+           __builtin_amdgcn_ballot_w64( !( fabs( p.y ) < 1.0 ) );
+}
+
 // One Horner step of be_multipole/track.h:62-76 (dpx + i dpy) <- bal +
 // (dpx + i dpy)(x + i y).  The exact build keeps the reference's order of
 // operations; the FMA build folds each component into two fused
 // multiply-adds: fma(dpx, x, fma(-dpy, y, bal_re)).
+#if defined( ST_FMA_BUILD )
+// a * b + c and -a * b + c with c a wave-uniform (SGPR) value.  Written as
+// one VOP3 v_fma_f64 reading c from its SGPR pair; left to itself the
+// compiler selects the two-address v_fmac_f64 and first copies c into the
+// destination VGPRs (two extra v_mov per call).
+ST_DEV double fma_sc( double const a, double const b, double const c )
+{
+#if defined( __HIP_DEVICE_COMPILE__ )
+    double d;
+    asm( "v_fma_f64 %0, %1, %2, %3" : "=v"( d ) : "v"( a ), "v"( b ), "s"( c ) );
+    return d;
+#else
+    return fma( a, b, c );
+#endif
+}
+
+ST_DEV double fnma_sc( double const a, double const b, double const c )
+{
+#if defined( __HIP_DEVICE_COMPILE__ )
+    double d;
+    asm( "v_fma_f64 %0, -%1, %2, %3" : "=v"( d ) : "v"( a ), "v"( b ), "s"( c ) );
+    return d;
+#else
+    return fma( -a, b, c );
+#endif
+}
+#endif
+
 ST_DEV void horner_step( double& dpx, double& dpy, double const x,
                          double const y, double const bre, double const bim )
 {
 #if defined( ST_FMA_BUILD )
-    double const ndpx = fma( dpx, x, fma( -dpy, y, bre ) );
-    dpy = fma( dpx, y, fma( dpy, x, bim ) );
+    double const ndpx = fma( dpx, x, fnma_sc( dpy, y, bre ) );
+    dpy = fma( dpx, y, fma_sc( dpy, x, bim ) );
     dpx = ndpx;
 #else
     double const zre = dpx * x - dpy * y;
@@ -150,34 +202,40 @@ ST_DEV void horner( cptr_t e, int const order, double const x, double const y,
 }
 
 // be_multipole/track.h:38-112
-ST_DEV void multipole( Lane& p, const LaneConst& c, cptr_t e, unsigned aux,
+ST_DEV void multipole( Lane& p, double const chi, cptr_t e, unsigned aux,
                        double const delta )
 {
     int const order = ( int )( aux & 0x7fffu );
     double dpx, dpy;
     double const x = p.x, y = p.y;
     horner( e, order, x, y, dpx, dpy );
-    dpx = -c.chi * dpx;
-    dpy = c.chi * dpy;
+    dpx = -chi * dpx;
+    dpy = chi * dpy;
     if( aux & ST_MP_CURV_FLAG )
     {
         int const nb = 2 * order + 2;
         double const hxl = rd( e, nb ), hyl = rd( e, nb + 1 ), length = rd( e, nb + 2 );
         double const hxlx = x * hxl;
         double const hyly = y * hyl;
-        p.zeta += c.chi * ( hyly - hxlx );
+        p.zeta += chi * ( hyly - hxlx );
         dpx += hxl + hxl * delta;
         dpy -= hyl + hyl * delta;
         if( length > 0.0 )
         {
-            double const b1l = c.chi * rd( e, 0 );
-            double const a1l = c.chi * rd( e, 1 );
+            double const b1l = chi * rd( e, 0 );
+            double const a1l = chi * rd( e, 1 );
             dpx -= b1l * hxlx / length;
             dpy += a1l * hyly / length;
         }
     }
     p.px += dpx;
     p.py += dpy;
+}
+
+ST_DEV void multipole( Lane& p, const LaneConst& c, cptr_t e, unsigned aux,
+                       double const delta )
+{
+    multipole( p, c.chi, e, aux, delta );
 }
 
 // particles.h:5255-5297 (energy0 5220-5234, zeta scaling 5024-5032)

@@ -212,8 +212,9 @@ struct NS(TrackJobBase)
         const unsigned char* eb = NS(Buffer_get_const_data_begin)( ebuf );
         int64_t const nw = NS(Lattice_pack)( eb, nullptr, 0, nullptr, &n_elem_l, nullptr, &n_mon );
         if( nw < 0 ) { g_err = "lattice could not be packed"; return false; }
-        // zero padding: the kernels read the header one record ahead
-        lat_host.assign( nw + 16, 0u );
+        // zero padding: the kernels read the header one record ahead and
+        // fetch multipole coefficients in batches of up to 16 words
+        lat_host.assign( nw + 32, 0u );
         rec_offsets.assign( n_elem_l + 1, 0u );
         monitor_fields.assign( n_mon, 0u );
         if( NS(Lattice_pack)( eb, lat_host.data(), nw, rec_offsets.data(), &n_elem_l,
@@ -224,7 +225,7 @@ struct NS(TrackJobBase)
         for( int64_t i = 0; i < n_elem; ++i )
         {
             unsigned const op = ST_REC_OP( lat_host[ rec_offsets[ i ] ] );
-            if( op > ST_OP_DIPEDGE || op == 0 ) lat_full = true;
+            if( !ST_OP_IS_LEAN( op ) ) lat_full = true;
         }
         // point the monitors that have an output set at its device header
         if( plan.monitor_elems.size() > 0 && d_obuf != nullptr )
@@ -234,7 +235,7 @@ struct NS(TrackJobBase)
             for( int64_t i = 0; i < n_elem && m < plan.monitor_elems.size(); ++i )
             {
                 if( ST_REC_OP( lat_host[ rec_offsets[ i ] ] ) != ST_OP_MONITOR ) continue;
-                uint32_t const field = rec_offsets[ i ] + 4u;
+                uint32_t const field = rec_offsets[ i ] + ST_REC_PAYLOAD + 3u;
                 if( i == plan.monitor_elems[ m ] )
                 {
                     NS(Object) const* o = NS(Buffer_get_const_object)( obuf, monitor_offset + m );

@@ -244,7 +244,7 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
             cptr_t const nr = r + ST_REC_LEN( h );
             uint64_t const hn = nr[ 0 ];          // next header (stream is padded)
             unsigned const op = ST_REC_OP( h );
-            cptr_t const prm = r + 1;
+            cptr_t const prm = r + ST_REC_PAYLOAD;
             if( MODE == MODE_EBE )
             {
                 int64_t const k = ebe_index( a.cfg, pid, start + e, at_turn );
@@ -303,12 +303,176 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
     if( in_range && status != 0 ) atomicOr( a.status_out, status );
 }
 
+// ------------------------------------------------- lean until-turn kernel
+//
+// Lattices made only of Drift, DriftExact, Multipole, Cavity, XYShift,
+// SRotation, the three apertures and DipoleEdge (every record op at most
+// ST_OP_DIPEDGE) run through this loop.  Same results as track_body<
+// MODE_UNTIL> (track/track.h:610-670); the structure is what differs:
+//   * the element loop is two nested loops: an inner one over runs of Drift
+//     and Multipole records (96 % of a typical ring), and one visit of the
+//     outer loop per other record.  Only the inner loop's two maps merge
+//     into the loop-carried lane state, so that state stays in place in the
+//     same registers instead of being copied at every element;
+//   * only chi is held in a register; the other per-particle constants are
+//     read from memory by the (rare) Cavity;
+//   * the particle state is not carried: an active lane is alive (state 1)
+//     and a stored loss writes state 0.
+
+// Store of a finished lane.  The index goes through an empty asm statement
+// so that the field addresses are formed on this (rare) path instead of
+// being kept live in registers across the element loop.
+ST_DEV void store_lane_state( const st_dev_particles_t& P, int64_t i, const Lane& L,
+                              int64_t state, int64_t at_element, int64_t at_turn )
+{
+    int64_t j = i;
+    asm volatile( "" : "+v"( j ) );
+    P.x[ j ] = L.x; P.px[ j ] = L.px; P.y[ j ] = L.y; P.py[ j ] = L.py;
+    P.zeta[ j ] = L.zeta; P.delta[ j ] = L.delta; P.rpp[ j ] = L.rpp;
+    P.rvv[ j ] = L.rvv; P.psigma[ j ] = L.psigma; P.s[ j ] = L.s;
+    P.state[ j ] = state;
+    P.at_element[ j ] = at_element;
+    P.at_turn[ j ] = at_turn;
+}
+
+// Records other than Drift / Multipole in a lean lattice.  Returns the
+// survival predicate; an op outside the lean set is a map failure
+// (track.h:577-581: lost, status -1).
+ST_DEV bool lean_other( unsigned op, cptr_t e, Lane& L, const st_dev_particles_t& P,
+                        int64_t li, bool& fail )
+{
+    switch( op )
+    {
+    case ST_OP_DRIFT_EXACT: drift_exact( L, rd( e, 0 ) ); return limit_global_ok( L );
+    case ST_OP_CAVITY:
+    {
+        int64_t j = li;
+        asm volatile( "" : "+v"( j ) );
+        LaneConst C;
+        C.q0 = P.q0[ j ]; C.mass0 = P.mass0[ j ]; C.beta0 = P.beta0[ j ];
+        C.p0c = P.p0c[ j ]; C.chi = P.chi[ j ]; C.charge_ratio = P.charge_ratio[ j ];
+        cavity( L, C, e );
+        return true;
+    }
+    case ST_OP_XYSHIFT: xyshift( L, e ); return true;
+    case ST_OP_SROTATION: srotation( L, e ); return true;
+    case ST_OP_LIMIT_RECT: return limit_rect_ok( L, e );
+    case ST_OP_LIMIT_ELLIPSE: return limit_ellipse_ok( L, e );
+    case ST_OP_LIMIT_RECT_ELLIPSE: return limit_rect_ellipse_ok( L, e );
+    case ST_OP_DIPEDGE: dipedge( L, e ); return true;
+    default: fail = true; return false;
+    }
+}
+
+// Record at byte offset ``off`` of the stream (32-bit offsets: one scalar
+// add per element and the offset form of the scalar load).
+ST_DEV cptr_t rec_at( cptr_t lat, uint32_t off )
+{
+    return ( cptr_t )( ( const __attribute__( ( address_space( 4 ) ) ) char* )lat + off );
+}
+
+ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t i, const Args& a )
+{
+    bool const in_range = i < P.n;
+    int64_t const li = in_range ? i : 0;          // out-of-range lanes never store
+    Lane L;
+    L.x = P.x[ li ]; L.px = P.px[ li ]; L.y = P.y[ li ]; L.py = P.py[ li ];
+    L.zeta = P.zeta[ li ]; L.delta = P.delta[ li ]; L.rpp = P.rpp[ li ];
+    L.rvv = P.rvv[ li ]; L.psigma = P.psigma[ li ]; L.s = P.s[ li ];
+    double const chi = P.chi[ li ];
+    int64_t const start = P.at_element[ li ];
+    int64_t at_turn = P.at_turn[ li ];
+    bool const active0 = in_range && a.n_elem > 0 && a.until_turn > at_turn &&
+                         P.state[ li ] == 1;
+    int32_t status = ( in_range && a.n_elem <= 0 ) ? -1 : 0;
+
+    // A lane leaves ``act`` once, after its final state has been stored.  A
+    // wave whose lanes are all lost finishes the turn without effect.
+    uint64_t act = ballot( active0 );
+    while( act != 0u )
+    {
+        uint32_t off = 0;
+        uint64_t h = rec_at( a.lat, 0 )[ 0 ];
+        for( ;; )
+        {
+#if defined( ST_FMA_BUILD )
+            double const rvvm1 = L.rvv - 1.0;         // changes only at a Cavity
+#endif
+            while( ( ( uint32_t )h & ST_OP_RARE_BIT ) == 0u )   // Drift / Multipole run
+            {
+                cptr_t const r = rec_at( a.lat, off );
+                uint32_t const noff = off + ST_REC_BYTES( h );
+                uint64_t const hn = rec_at( a.lat, noff )[ 0 ];
+                if( ( uint32_t )h & ST_OP_MP_BIT )
+                {
+                    multipole( L, chi, r + ST_REC_PAYLOAD, ST_REC_AUX( h ), L.delta );
+                }
+                else
+                {
+#if defined( ST_FMA_BUILD )
+                    drift_rvvm1( L, rd( r, ST_REC_PAYLOAD ), rvvm1 );
+#else
+                    drift( L, rd( r, ST_REC_PAYLOAD ) );
+#endif
+                    uint64_t const lost = limit_global_lost_mask( L ) & act;
+                    if( lost != 0u )
+                    {
+                        if( lane_in( lost ) )
+                            store_lane_state( P, i, L, 0, start + ri( r, 1 ), at_turn );
+                        act &= ~lost;
+                    }
+                }
+                off = noff;
+                h = hn;
+            }
+            unsigned const op = ST_REC_OP( h );
+            if( op == ST_OP_END ) break;
+            cptr_t const r = rec_at( a.lat, off );
+            uint32_t const noff = off + ST_REC_BYTES( h );
+            uint64_t const hn = rec_at( a.lat, noff )[ 0 ];
+            bool fail = false;
+            bool const ok = lean_other( op, r + ST_REC_PAYLOAD, L, P, li, fail );
+            uint64_t const lost = ballot( !ok ) & act;
+            if( lost != 0u )
+            {
+                if( lane_in( lost ) )
+                {
+                    store_lane_state( P, i, L, 0, start + ri( r, 1 ), at_turn );
+                    if( fail ) status = -1;
+                }
+                act &= ~lost;
+            }
+            off = noff;
+            h = hn;
+        }
+        if( lane_in( act ) ) { ++at_turn; L.s = 0.0; }
+        uint64_t const fin = ballot( at_turn >= a.until_turn ) & act;
+        if( fin != 0u )
+        {
+            if( lane_in( fin ) ) store_lane_state( P, i, L, 1, start, at_turn );
+            act &= ~fin;
+        }
+    }
+    if( in_range && status != 0 ) atomicOr( a.status_out, status );
+}
+
 }  // namespace
 
-template< bool FULL >
 __global__ void __launch_bounds__( 256 ) KN( st_k_track_until )( st_dev_particles_t P, Args a )
 {
-    track_body< MODE_UNTIL, FULL >( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
+    track_body< MODE_UNTIL, true >( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
+}
+
+// Occupancy target of the lean kernel (waves per SIMD): the register
+// budget it implies spills only on the Cavity and store paths.
+#ifndef ST_LEAN_WAVES_PER_EU
+#define ST_LEAN_WAVES_PER_EU 8
+#endif
+
+__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
+KN( st_k_track_until_lean )( st_dev_particles_t P, Args a )
+{
+    track_until_lean( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
 }
 
 __global__ void __launch_bounds__( 256 ) KN( st_k_track_ebe )( st_dev_particles_t P, Args a )
@@ -343,10 +507,10 @@ extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
     a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
     a.status_out = status;
     if( full )
-        hipLaunchKernelGGL( KN( st_k_track_until )< true >, dim3( grid_for( P->n, block ) ),
+        hipLaunchKernelGGL( KN( st_k_track_until ), dim3( grid_for( P->n, block ) ),
                             dim3( block ), 0, stream, *P, a );
     else
-        hipLaunchKernelGGL( KN( st_k_track_until )< false >, dim3( grid_for( P->n, block ) ),
+        hipLaunchKernelGGL( KN( st_k_track_until_lean ), dim3( grid_for( P->n, block ) ),
                             dim3( block ), 0, stream, *P, a );
     return hipGetLastError();
 }

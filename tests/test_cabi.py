@@ -90,15 +90,22 @@ def test_lattice_pack_all_element_types():
     assert st.st_Lattice_pack(eb.pointer, out.ctypes.data, nw, offs.ctypes.data,
                               ct.byref(n_el), None, ct.byref(n_mon)) == nw
     ops = [int(out[o]) & 0xff for o in offs[:-1]]
-    lens = [int(out[o]) >> 32 for o in offs[:-1]]
-    assert offs[-1] == nw and all(np.diff(offs) == lens)
-    # opcode per type id (lattice.h)
-    want = {2: 1, 3: 2, 4: 3, 5: 4, 6: 5, 7: 6, 11: 7, 12: 8, 16: 9, 24: 10, 10: 11,
-            8: 12, 9: 13, 13: 14, 14: 15, 257: 16, 256: 17}
+    nbytes = [int(out[o]) >> 32 for o in offs[:-1]]
+    # records are contiguous, lengths in bytes, the element index is word 1
+    assert all(np.diff(offs) * 8 == nbytes)
+    assert [int(out[o + 1]) for o in offs[:-1]] == list(range(n_el.value))
+    # end record closes the stream
+    assert offs[-1] == nw - 2 and int(out[nw - 2]) & 0xff == 0xff
+    assert int(out[nw - 2]) >> 32 == 16 and int(out[nw - 1]) == n_el.value
+    # opcode per type id (lattice.h); Drift and Multipole are the only ones
+    # without the 0x80 bit
+    want = {2: 0x01, 3: 0x81, 4: 0x02, 5: 0x82, 6: 0x83, 7: 0x84, 11: 0x85, 12: 0x86,
+            16: 0x87, 24: 0x88, 10: 0x89, 8: 0x8a, 9: 0x8b, 13: 0x8c, 14: 0x8d,
+            257: 0x8e, 256: 0x8f}
     assert ops == [want[int(t)] for t in eb.typeids()]
     # cavity folding: (pi/180)*lag and ((2 pi)/c)*f
     k = list(eb.typeids()).index(5)
-    cav = out[offs[k] + 1: offs[k] + 4].view(np.float64)
+    cav = out[offs[k] + 2: offs[k] + 5].view(np.float64)
     assert cav[0] == 5e6
     assert cav[1] == (np.pi / 180.0) * 170.0
     assert cav[2] == ((2.0 * np.pi) / 299792458.0) * 200e6
