@@ -85,6 +85,31 @@ def cpu_baseline(eb, pb, budget_s=10.0, sample=8192, turns_per_chunk=2):
                                    "x %d turns of the same ring, %.1f s" % (len(idx), n, turn, dt))
 
 
+def calibration_copy(nbytes):
+    """One device-to-device copy of ``nbytes`` through the HIP runtime (its
+    blit kernel reads and writes exactly nbytes of HBM)."""
+    hip = ct.CDLL("libamdhip64.so")
+    a, b = ct.c_void_p(), ct.c_void_p()
+    assert hip.hipMalloc(ct.byref(a), ct.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(ct.byref(b), ct.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, ct.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(b, a, ct.c_size_t(nbytes), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -98,6 +123,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-steps", type=int, default=1,
                     help="extra timed steps with the exact (no-FMA) build, reported beside")
+    ap.add_argument("--calibrate-copy-mib", type=int, default=0,
+                    help="also run one device-to-device hipMemcpy of this many MiB (a copy "
+                         "kernel of known traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -139,6 +167,8 @@ def main():
         return tracked, t1 - t0, kernel_ms, alive
 
     setup_s = time.perf_counter() - t_setup
+    if args.calibrate_copy_mib > 0:
+        calibration_copy(args.calibrate_copy_mib << 20)
     tracked, wall, kernel_ms, alive = run(args.mode, args.steps, args.warmup)
     value = tracked / wall
     kernel_s = kernel_ms / 1e3
@@ -158,7 +188,7 @@ def main():
     if not args.no_cpu_baseline:
         v, threads, desc = cpu_baseline(eb, pb, budget_s=args.cpu_budget)
         cpu = {"value": v, "unit": "particle-turns/s", "cores": threads, "kind": "port",
-               "sample": desc}
+               "sample": desc, "cpu_model": cpu_model()}
 
     ms_per_step = wall * 1e3 / max(1, args.steps)
     out = {

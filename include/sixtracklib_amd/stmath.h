@@ -22,8 +22,11 @@
  *   sin, cos : Cody-Waite reduction by pi/2 with a three-part constant
  *              (33 + 33 + 53 bits of pi/2) -> |r| <= pi/4, then Taylor
  *              polynomials to r^19 (sin) / r^20 (cos); truncation < 1e-19.
- *              The reduction is exact-enough for |x| < 2^20 * pi/2; beyond
- *              that the result degrades gracefully (documented limit).
+ *              k * (first two parts) is exact for |x| < 2^20 * pi/2; up
+ *              to |x| < 2^30 the reduced argument is still good to
+ *              |x| * 2^-53 absolute.  Beyond 2^30 (or for Inf / NaN) the
+ *              result is NaN: a documented domain limit (libm would return
+ *              a finite value; a tracking phase of 1e9 rad is not physical).
  *   exp      : k = rint(x / ln 2), two-part ln 2, Taylor to r^14 on
  *              |r| <= ln2/2, scaled with ldexp (exact, handles subnormals).
  *   log      : frexp into m in [sqrt(1/2), sqrt(2)), log m = 2 atanh(s),
@@ -89,48 +92,44 @@ ST_MATH_FN double st_cos_kernel( double const r )
     return w + ( ( ( 1.0 - w ) - hz ) + ( z * z ) * q );
 }
 
-/* Reduce x by multiples of pi/2; returns r and the quadrant (k mod 4). */
+/* Reduce x by multiples of pi/2; returns r and the quadrant (k mod 4).
+ * k mod 4 is formed in floating point (k/4, floor and the products are
+ * exact for every finite k), so no out-of-range integer conversion can
+ * occur. */
 ST_MATH_FN double st_reduce_pio2( double const x, int* quadrant )
 {
     double const k = rint( x * ST_TWO_OVER_PI );
     double r = x - k * ST_PIO2_1;
     r = r - k * ST_PIO2_2;
     r = r - k * ST_PIO2_3;
-    long long const ki = ( long long )k;
-    *quadrant = ( int )( ki & 3 );
+    *quadrant = ( int )( k - 4.0 * floor( 0.25 * k ) );
     return r;
 }
 
+/* Both polynomials are evaluated and the quadrant selects between them:
+ * no data-dependent branch (a wave evaluates each polynomial once), and
+ * the selected value is the same IEEE result a branch would produce.
+ * Outside the domain (|x| >= 2^30, Inf, NaN) the result is NaN. */
 ST_MATH_FN double st_sin( double const x )
 {
-    if( !( fabs( x ) < ( double )INFINITY ) ) return x - x; /* NaN, Inf */
+    int const finite = fabs( x ) < 0x1p+30;   /* domain; false for Inf / NaN */
     int q = 0;
-    double const r = st_reduce_pio2( x, &q );
-    double res;
-    switch( q )
-    {
-        case 0:  res =  st_sin_kernel( r ); break;
-        case 1:  res =  st_cos_kernel( r ); break;
-        case 2:  res = -st_sin_kernel( r ); break;
-        default: res = -st_cos_kernel( r );
-    }
-    return res;
+    double const r = st_reduce_pio2( finite ? x : 0.0, &q );
+    double const s = st_sin_kernel( r ), c = st_cos_kernel( r );
+    double const v = ( q & 1 ) ? c : s;
+    double const res = ( q & 2 ) ? -v : v;
+    return finite ? ( x == 0.0 ? x : res ) : ( double )NAN;   /* sin(-0) = -0 */
 }
 
 ST_MATH_FN double st_cos( double const x )
 {
-    if( !( fabs( x ) < ( double )INFINITY ) ) return x - x;
+    int const finite = fabs( x ) < 0x1p+30;
     int q = 0;
-    double const r = st_reduce_pio2( x, &q );
-    double res;
-    switch( q )
-    {
-        case 0:  res =  st_cos_kernel( r ); break;
-        case 1:  res = -st_sin_kernel( r ); break;
-        case 2:  res = -st_cos_kernel( r ); break;
-        default: res =  st_sin_kernel( r );
-    }
-    return res;
+    double const r = st_reduce_pio2( finite ? x : 0.0, &q );
+    double const s = st_sin_kernel( r ), c = st_cos_kernel( r );
+    double const v = ( q & 1 ) ? s : c;
+    double const res = ( ( q + 1 ) & 2 ) ? -v : v;
+    return finite ? res : ( double )NAN;
 }
 
 ST_MATH_FN double st_exp( double const x )

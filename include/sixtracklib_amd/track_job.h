@@ -20,7 +20,12 @@
  * config_str tokens (space or comma separated): "fma=0|1" (default 0:
  * results bit-identical to the reference CPU tracker; 1: fused
  * multiply-add build, within 1e-13 relative), "block=64..256",
- * "device=N", "ebe_rolling=0|1" (default 1, as the reference job).
+ * "device=N", "ebe_rolling=0|1" (default 1, as the reference job),
+ * "compact=0|1" (default 1: track_until first gathers the particles that
+ * still have turns to go, so lost ones take no lanes; results identical),
+ * "turns_per_launch=K" (default 0: chosen from the work size; with
+ * compact=1 a request is split into launches of K turns and the particle
+ * list is rebuilt between them; results identical).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_JOB_H__
 #define SIXTRACKLIB_AMD_TRACK_JOB_H__

@@ -55,6 +55,11 @@ def _jobs():
             # the element dispatch branches only on wave-uniform values:
             # leave those regions unstructurized (no flow blocks / copies)
             defs += ["-mllvm", "-structurizecfg-skip-uniform-regions=true"]
+            # no machine-level hoisting of constants out of the element loop:
+            # at the 64-VGPR budget of the lean kernel the hoisted
+            # polynomial constants of sin/exp would be spilled to scratch
+            # and reloaded at every Cavity (rematerialising them is free)
+            defs += ["-mllvm", "-disable-machine-licm"]
             j.append(([HIPCC] + HIP_COMMON + defs + ["-ffp-contract=" + contract,
                                                      "-DST_KSUFFIX=" + variant, "-c",
                                               os.path.join(CSRC, src), "-o", out],

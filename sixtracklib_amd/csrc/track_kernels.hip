@@ -201,6 +201,9 @@ struct Args
     st_ebe_config_t cfg;     // ebe mode
     st_dev_particles_t O;    // ebe output set
     int32_t* status_out;
+    const int32_t* list;     // lean until: particle indices to track (or null)
+    int64_t n_list;
+    int64_t max_turns;       // lean until: turns per lane in this launch (<= 0: all)
 };
 
 // Per-particle loops: track/track.h:610-670 (until), 672-777 (element by
@@ -371,9 +374,12 @@ ST_DEV cptr_t rec_at( cptr_t lat, uint32_t off )
     return ( cptr_t )( ( const __attribute__( ( address_space( 4 ) ) ) char* )lat + off );
 }
 
-ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t i, const Args& a )
+ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t tid, const Args& a )
 {
-    bool const in_range = i < P.n;
+    // with a list, lane tid tracks particle list[tid] (the particles still
+    // to be tracked, gathered by st_k_select_active); else particle tid
+    bool const in_range = ( a.list != nullptr ) ? tid < a.n_list : tid < P.n;
+    int64_t const i = ( a.list != nullptr && in_range ) ? ( int64_t )a.list[ tid ] : tid;
     int64_t const li = in_range ? i : 0;          // out-of-range lanes never store
     Lane L;
     L.x = P.x[ li ]; L.px = P.px[ li ]; L.y = P.y[ li ]; L.py = P.py[ li ];
@@ -384,6 +390,10 @@ ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t i, const Args
     int64_t at_turn = P.at_turn[ li ];
     bool const active0 = in_range && a.n_elem > 0 && a.until_turn > at_turn &&
                          P.state[ li ] == 1;
+    // a launch may cover only part of the requested turns; the lane then
+    // stops at a turn boundary exactly as if until_turn were turn_stop
+    int64_t const turn_stop = ( a.max_turns > 0 && a.until_turn - at_turn > a.max_turns )
+                                  ? at_turn + a.max_turns : a.until_turn;
     int32_t status = ( in_range && a.n_elem <= 0 ) ? -1 : 0;
 
     // A lane leaves ``act`` once, after its final state has been stored.  A
@@ -446,7 +456,7 @@ ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t i, const Args
             h = hn;
         }
         if( lane_in( act ) ) { ++at_turn; L.s = 0.0; }
-        uint64_t const fin = ballot( at_turn >= a.until_turn ) & act;
+        uint64_t const fin = ballot( at_turn >= turn_stop ) & act;
         if( fin != 0u )
         {
             if( lane_in( fin ) ) store_lane_state( P, i, L, 1, start, at_turn );
@@ -457,6 +467,35 @@ ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t i, const Args
 }
 
 }  // namespace
+
+// Indices of the particles a track_until call has work for (state 1 and
+// at_turn < until_turn), appended wave by wave to ``list``: one atomic add
+// per wave reserves the wave's slots.  The order of the list is not the
+// particle order; every particle is tracked independently, so the results
+// do not depend on it.
+// count[1] receives the smallest at_turn among them (clamped to int32).
+__global__ void __launch_bounds__( 256 ) KN( st_k_select_active )( st_dev_particles_t P,
+    int64_t until_turn, int32_t* list, int32_t* count )
+{
+    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t const t = ( i < P.n ) ? P.at_turn[ i ] : 0;
+    bool const want = i < P.n && P.state[ i ] == 1 && t < until_turn;
+    uint64_t const m = ballot( want );
+    if( m == 0u ) return;
+    int32_t const t32 = ( int32_t )( want ? ( t < 0 ? 0 : ( t > INT32_MAX ? INT32_MAX : t ) )
+                                          : INT32_MAX );
+    int32_t tmin = t32;
+    for( int off = 32; off > 0; off >>= 1 ) tmin = min( tmin, __shfl_xor( tmin, off ) );
+    int base = 0;
+    if( __lane_id() == 0 )
+    {
+        base = atomicAdd( &count[ 0 ], __popcll( m ) );
+        atomicMin( &count[ 1 ], tmin );
+    }
+    base = __shfl( base, 0 );
+    uint64_t const below = ( __lane_id() == 0 ) ? 0u : ( m << ( 64 - __lane_id() ) );
+    if( want ) list[ base + __popcll( below ) ] = ( int32_t )i;
+}
 
 __global__ void __launch_bounds__( 256 ) KN( st_k_track_until )( st_dev_particles_t P, Args a )
 {
@@ -497,9 +536,19 @@ static inline int sane_block( int block )
     return ( block <= 0 || block > 256 || ( block % 64 ) != 0 ) ? 256 : block;
 }
 
+extern "C" hipError_t KN( st_launch_select_active )( const st_dev_particles_t* P,
+    int64_t until_turn, int32_t* list, int32_t* count, hipStream_t stream )
+{
+    if( P->n <= 0 ) return hipSuccess;
+    hipLaunchKernelGGL( KN( st_k_select_active ), dim3( grid_for( P->n, 256 ) ), dim3( 256 ),
+                        0, stream, *P, until_turn, list, count );
+    return hipGetLastError();
+}
+
 extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
     const uint64_t* lat, int64_t n_elem, int64_t until_turn, int full,
-    int32_t* status, int block, hipStream_t stream )
+    const int32_t* list, int64_t n_list, int64_t max_turns, int32_t* status, int block,
+    hipStream_t stream )
 {
     if( P->n <= 0 ) return hipSuccess;
     block = sane_block( block );
@@ -507,11 +556,18 @@ extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
     a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
     a.status_out = status;
     if( full )
+    {
         hipLaunchKernelGGL( KN( st_k_track_until ), dim3( grid_for( P->n, block ) ),
                             dim3( block ), 0, stream, *P, a );
+    }
     else
-        hipLaunchKernelGGL( KN( st_k_track_until_lean ), dim3( grid_for( P->n, block ) ),
+    {
+        int64_t const n = ( list != nullptr ) ? n_list : P->n;
+        if( list != nullptr && ( n_list <= 0 || n_list > P->n ) ) return hipSuccess;
+        a.list = list; a.n_list = n_list; a.max_turns = max_turns;
+        hipLaunchKernelGGL( KN( st_k_track_until_lean ), dim3( grid_for( n, block ) ),
                             dim3( block ), 0, stream, *P, a );
+    }
     return hipGetLastError();
 }
 

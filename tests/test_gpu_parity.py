@@ -93,10 +93,17 @@ def test_driftexact_fixture_bit_exact_on_gpu():
     assert_particles_equal(got, {f: g["final_" + f] for f in ALL_FIELDS}, what="driftexact")
 
 
-@pytest.mark.parametrize("turns", [1, 20])
-def test_fodo_ring_exact_bitwise(turns):
+@pytest.mark.parametrize("turns,sigma", [(1, 1e-3), (20, 1e-3), (100, 1e-3), (100, 4e-4)])
+def test_fodo_ring_exact_bitwise(turns, sigma):
+    """100 turns of 1,000 particles through the 402-element FODO ring is the
+    CPU-runnable reference configuration, compared in full.  Its 100
+    sextupoles leave a dynamic aperture of about 0.4 mm at this beta, so at
+    sigma = 1 mm most particles are lost within 5 turns (the comparison then
+    checks loss element, turn and frozen state); sigma = 0.4 mm keeps about
+    half of them for all 100 turns."""
     eb = lattices.fodo_ring()
-    pb, _ = lattices.gaussian_beam(1000, seed=0)
+    pb, _ = lattices.gaussian_beam(1000, seed=0, sigma_x=sigma, sigma_y=sigma,
+                                   sigma_px=sigma / 100, sigma_py=sigma / 100)
     ob = _copy(pb)
     assert oracle_lib.track_until(ob, eb, turns) == 0
     got = _gpu_until(eb, pb, turns)
@@ -124,6 +131,32 @@ def test_fodo_ring_losses_match_oracle():
     want = _pdict(ob)
     assert np.count_nonzero(want["state"] == 0) > 100
     assert_particles_equal(got, want, what="fodo losses")
+
+
+def test_fodo_ring_losses_in_steps_and_compaction():
+    """track_until in steps (3, 7, 10 turns) gathers the still-active
+    particles before each launch; the result equals one oracle call to 10
+    turns bit for bit, with the gathering off, and with every request split
+    into launches of 2 turns (list rebuilt between them)."""
+    eb = lattices.fodo_ring()
+    pb, _ = lattices.gaussian_beam(3000, seed=5, sigma_x=4e-4, sigma_y=4e-4,
+                                   sigma_px=4e-6, sigma_py=4e-6)
+    ob, pb2, pb3 = _copy(pb), _copy(pb), _copy(pb)
+    oracle_lib.track_until(ob, eb, 10)
+    from sixtracklib_amd.trackjob import TrackJob
+    got = {}
+    for key, buf, cfg in (("on", pb, "fma=0"), ("off", pb2, "fma=0 compact=0"),
+                          ("chunked", pb3, "fma=0 turns_per_launch=2")):
+        job = TrackJob(eb, buf, config_str=cfg)
+        for t in (3, 7, 10):
+            assert job.track_until(t) == 0
+        job.collect()
+        got[key] = _pdict(buf)
+    want = _pdict(ob)
+    assert 100 < np.count_nonzero(want["state"] == 0) < 2900
+    assert_particles_equal(got["on"], want, what="stepped, gathered")
+    assert_particles_equal(got["off"], want, what="stepped, not gathered")
+    assert_particles_equal(got["chunked"], want, what="stepped, 2 turns per launch")
 
 
 @pytest.mark.parametrize("seed", [0, 1])
