@@ -21,11 +21,14 @@ __all__ = ["CBuffer", "CObject", "CField", "get_object", "Particles", "Particles
            "SRotation", "BeamMonitor", "LimitRect", "LimitEllipse", "LimitRectEllipse",
            "DipoleEdge", "BeamBeam4D", "BeamBeam6D", "SCCoasting", "SCQGaussProfile",
            "SCInterpolatedProfile", "LineDensityProfileData", "Elements",
-           "append_beam_monitors_to_lattice", "TrackJob"]
+           "append_beam_monitors_to_lattice", "TrackJob", "TrackBatch"]
 
 
 def __getattr__(name):
     if name == "TrackJob":
         from .trackjob import TrackJob
         return TrackJob
+    if name == "TrackBatch":
+        from .trackbatch import TrackBatch
+        return TrackBatch
     raise AttributeError(name)

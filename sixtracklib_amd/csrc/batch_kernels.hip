@@ -1,0 +1,163 @@
+// Batched multi-seed tracking kernels for gfx950 (compiled twice, like
+// track_kernels.hip: _exact without FP contraction, _fma with it).
+//
+// No counterpart in the reference: its CPU job loops over particle sets
+// one after the other (internal/track_job_cpu.cpp:360-376) and its CUDA
+// job tracks a single set (cuda/track/track_job.cpp:2347).  Every lane
+// runs the same per-particle map as the plain lean kernel
+// (track_lean_lane), so each seed's results are those of a plain
+// track_until on that seed alone.
+//
+// One chunk of turns is three launches:
+//   reset   queue counters and per-seed counts;
+//   select  per seed, the particles with turns to go -> the seed's list;
+//   track   a persistent grid: every wave takes 64-particle tiles from its
+//           queue (one queue per XCD when seeds are grouped, so the waves
+//           of an XCD share one or two lattices in its L2), tracks a tile
+//           for up to max_turns turns, takes the next; when its queue is
+//           empty it takes tiles from the other queues.  Tiles are finite
+//           and counters only grow, so every wave reaches the exit.
+#include <hip/hip_runtime.h>
+
+#include "track_device.h"
+#include "track_lean.h"
+#include "batch.h"
+
+#ifndef ST_KSUFFIX
+#error "ST_KSUFFIX (kernel name suffix of this build variant) must be defined"
+#endif
+#define ST_CAT2( a, b ) a##b
+#define ST_CAT( a, b ) ST_CAT2( a, b )
+#define KN( n ) ST_CAT( n, ST_KSUFFIX )
+
+#ifndef ST_LEAN_WAVES_PER_EU
+#define ST_LEAN_WAVES_PER_EU 8
+#endif
+
+using namespace st;
+
+namespace
+{
+// read-only per-launch tables through the constant address space: the
+// indices are wave-uniform, so these become scalar loads
+typedef __attribute__( ( address_space( 4 ) ) ) const int32_t ci32_t;
+
+static_assert( sizeof( st_dev_particles_t ) == 22 * 8, "particle view: count + 21 pointers" );
+static_assert( sizeof( st_batch_seed_t ) == 25 * 8, "seed descriptor: view, lat, n_elem, list" );
+
+// Seed descriptor k, read as 25 words through the constant address space.
+struct SeedView
+{
+    st_dev_particles_t P;
+    cptr_t lat;
+    int64_t n_elem;
+    int32_t* list;
+};
+
+ST_DEV SeedView load_seed( const st_batch_seed_t* seeds, int k )
+{
+    cptr_t const w = ( cptr_t )seeds + 25 * k;
+    SeedView v;
+    v.P.n = ( int64_t )w[ 0 ];
+    v.P.q0 = ( double* )w[ 1 ];      v.P.mass0 = ( double* )w[ 2 ];
+    v.P.beta0 = ( double* )w[ 3 ];   v.P.gamma0 = ( double* )w[ 4 ];
+    v.P.p0c = ( double* )w[ 5 ];     v.P.s = ( double* )w[ 6 ];
+    v.P.x = ( double* )w[ 7 ];       v.P.y = ( double* )w[ 8 ];
+    v.P.px = ( double* )w[ 9 ];      v.P.py = ( double* )w[ 10 ];
+    v.P.zeta = ( double* )w[ 11 ];   v.P.psigma = ( double* )w[ 12 ];
+    v.P.delta = ( double* )w[ 13 ];  v.P.rpp = ( double* )w[ 14 ];
+    v.P.rvv = ( double* )w[ 15 ];    v.P.chi = ( double* )w[ 16 ];
+    v.P.charge_ratio = ( double* )w[ 17 ];
+    v.P.particle_id = ( int64_t* )w[ 18 ]; v.P.at_element = ( int64_t* )w[ 19 ];
+    v.P.at_turn = ( int64_t* )w[ 20 ];     v.P.state = ( int64_t* )w[ 21 ];
+    v.lat = ( cptr_t )w[ 22 ];
+    v.n_elem = ( int64_t )w[ 23 ];
+    v.list = ( int32_t* )w[ 24 ];
+    return v;
+}
+}  // namespace
+
+__global__ void __launch_bounds__( 256 ) KN( st_k_batch_reset )( st_batch_sched_t s )
+{
+    int const i = blockIdx.x * blockDim.x + threadIdx.x;
+    int const stride = gridDim.x * blockDim.x;
+    for( int q = i; q < s.n_queues; q += stride ) s.queue_next[ q ] = 0;
+    for( int k = i; k < s.n_seeds; k += stride )
+    {
+        s.counts[ 2 * k ] = 0;
+        s.counts[ 2 * k + 1 ] = INT32_MAX;
+    }
+}
+
+__global__ void __launch_bounds__( 256 ) KN( st_k_batch_select )( st_batch_sched_t s,
+    int64_t until_turn )
+{
+    SeedView const S = load_seed( s.seeds, ( int )blockIdx.y );
+    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
+    if( ( int64_t )blockIdx.x * blockDim.x >= S.P.n ) return;      // whole block past the set
+    select_active_lane( S.P, i, until_turn, S.list, &s.counts[ 2 * blockIdx.y ] );
+}
+
+__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
+KN( st_k_batch_track )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
+                        int32_t* status )
+{
+    ci32_t* const counts = ( ci32_t* )s.counts;
+    ci32_t* const qseeds = ( ci32_t* )s.queue_seeds;
+    ci32_t* const qbegin = ( ci32_t* )s.queue_begin;
+    int const nq = s.n_queues;
+    int const q0 = ( int )( blockIdx.x % ( unsigned )nq );
+    for( int qi = 0; qi < nq; ++qi )
+    {
+        int const q = ( q0 + qi ) % nq;
+        int const qb = qbegin[ q ], qe = qbegin[ q + 1 ];
+        for( ;; )
+        {
+            int t = 0;
+            if( __lane_id() == 0 ) t = atomicAdd( &s.queue_next[ q ], 1 );
+            t = __builtin_amdgcn_readfirstlane( __shfl( t, 0 ) );
+            int seed = -1;
+            for( int k = qb; k < qe; ++k )
+            {
+                int const sd = qseeds[ k ];
+                int const nt = ( counts[ 2 * sd ] + 63 ) >> 6;
+                if( t < nt ) { seed = sd; break; }
+                t -= nt;
+            }
+            if( seed < 0 ) break;                    // this queue is drained
+            SeedView const S = load_seed( s.seeds, seed );
+            track_lean_lane( S.P, S.list, counts[ 2 * seed ], ( int64_t )t * 64 + __lane_id(),
+                             S.lat, S.n_elem, until_turn, max_turns, status );
+        }
+    }
+}
+
+// ------------------------------------------------------------ launchers
+
+extern "C" hipError_t KN( st_launch_batch_reset )( const st_batch_sched_t* s, hipStream_t stream )
+{
+    hipLaunchKernelGGL( KN( st_k_batch_reset ), dim3( 1 ), dim3( 256 ), 0, stream, *s );
+    return hipGetLastError();
+}
+
+extern "C" hipError_t KN( st_launch_batch_select )( const st_batch_sched_t* s,
+    int64_t max_particles, int64_t until_turn, hipStream_t stream )
+{
+    if( max_particles <= 0 || s->n_seeds <= 0 ) return hipSuccess;
+    unsigned const gx = ( unsigned )( ( max_particles + 255 ) / 256 );
+    hipLaunchKernelGGL( KN( st_k_batch_select ), dim3( gx, ( unsigned )s->n_seeds ), dim3( 256 ),
+                        0, stream, *s, until_turn );
+    return hipGetLastError();
+}
+
+extern "C" hipError_t KN( st_launch_batch_track )( const st_batch_sched_t* s,
+    int64_t until_turn, int64_t max_turns, int32_t* status, int block, int grid,
+    hipStream_t stream )
+{
+    if( s->n_seeds <= 0 || s->n_queues <= 0 ) return hipSuccess;
+    if( block <= 0 || block > 256 || ( block % 64 ) != 0 ) block = 256;
+    if( grid <= 0 ) grid = 1;
+    hipLaunchKernelGGL( KN( st_k_batch_track ), dim3( ( unsigned )grid ), dim3( ( unsigned )block ),
+                        0, stream, *s, until_turn, max_turns, status );
+    return hipGetLastError();
+}

@@ -35,6 +35,11 @@ constexpr double DEG2RAD_CONST = 0.017453292519943295769236907684886127134428718
 constexpr double QELEM = 1.60217662e-19;
 constexpr double CHARGE0_SI = 1.602176634e-19;
 
+// Lane bookkeeping lives in wave-uniform 64-bit masks (SGPRs): the vote of
+// a predicate and the test of this lane's bit in a mask.
+ST_DEV uint64_t ballot( bool p ) { return __builtin_amdgcn_ballot_w64( p ); }
+ST_DEV bool lane_in( uint64_t m ) { return ( ( m >> __lane_id() ) & 1u ) != 0u; }
+
 ST_DEV double rd( cptr_t p, int k ) { return __longlong_as_double( ( long long )p[ k ] ); }
 ST_DEV int64_t ri( cptr_t p, int k ) { return ( int64_t )p[ k ]; }
 

@@ -741,4 +741,7 @@ int NS(Hip_num_devices)( void )
 
 char const* NS(Hip_last_error_str)( void ) { return g_err.c_str(); }
 
+// library-internal: other translation units (batch.cpp) report errors here
+void st_internal_set_error( char const* msg ) { g_err = msg ? msg : ""; }
+
 }  // extern "C"

@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 
 #include "track_device.h"
+#include "track_lean.h"
 #include "track_launch.h"
 
 #ifndef ST_KSUFFIX
@@ -42,11 +43,6 @@ namespace
 {
 
 enum { MODE_UNTIL = 0, MODE_EBE = 1, MODE_LINE = 2 };
-
-// Lane bookkeeping lives in wave-uniform 64-bit masks (SGPRs): the vote of
-// a predicate and the test of this lane's bit in a mask.
-ST_DEV uint64_t ballot( bool p ) { return __builtin_amdgcn_ballot_w64( p ); }
-ST_DEV bool lane_in( uint64_t m ) { return ( ( m >> __lane_id() ) & 1u ) != 0u; }
 
 ST_DEV void load_lane( const st_dev_particles_t& P, int64_t i, Lane& L, LaneConst& C )
 {
@@ -306,195 +302,20 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
     if( in_range && status != 0 ) atomicOr( a.status_out, status );
 }
 
-// ------------------------------------------------- lean until-turn kernel
-//
-// Lattices made only of Drift, DriftExact, Multipole, Cavity, XYShift,
-// SRotation, the three apertures and DipoleEdge (every record op at most
-// ST_OP_DIPEDGE) run through this loop.  Same results as track_body<
-// MODE_UNTIL> (track/track.h:610-670); the structure is what differs:
-//   * the element loop is two nested loops: an inner one over runs of Drift
-//     and Multipole records (96 % of a typical ring), and one visit of the
-//     outer loop per other record.  Only the inner loop's two maps merge
-//     into the loop-carried lane state, so that state stays in place in the
-//     same registers instead of being copied at every element;
-//   * only chi is held in a register; the other per-particle constants are
-//     read from memory by the (rare) Cavity;
-//   * the particle state is not carried: an active lane is alive (state 1)
-//     and a stored loss writes state 0.
-
-// Store of a finished lane.  The index goes through an empty asm statement
-// so that the field addresses are formed on this (rare) path instead of
-// being kept live in registers across the element loop.
-ST_DEV void store_lane_state( const st_dev_particles_t& P, int64_t i, const Lane& L,
-                              int64_t state, int64_t at_element, int64_t at_turn )
-{
-    int64_t j = i;
-    asm volatile( "" : "+v"( j ) );
-    P.x[ j ] = L.x; P.px[ j ] = L.px; P.y[ j ] = L.y; P.py[ j ] = L.py;
-    P.zeta[ j ] = L.zeta; P.delta[ j ] = L.delta; P.rpp[ j ] = L.rpp;
-    P.rvv[ j ] = L.rvv; P.psigma[ j ] = L.psigma; P.s[ j ] = L.s;
-    P.state[ j ] = state;
-    P.at_element[ j ] = at_element;
-    P.at_turn[ j ] = at_turn;
-}
-
-// Records other than Drift / Multipole in a lean lattice.  Returns the
-// survival predicate; an op outside the lean set is a map failure
-// (track.h:577-581: lost, status -1).
-ST_DEV bool lean_other( unsigned op, cptr_t e, Lane& L, const st_dev_particles_t& P,
-                        int64_t li, bool& fail )
-{
-    switch( op )
-    {
-    case ST_OP_DRIFT_EXACT: drift_exact( L, rd( e, 0 ) ); return limit_global_ok( L );
-    case ST_OP_CAVITY:
-    {
-        int64_t j = li;
-        asm volatile( "" : "+v"( j ) );
-        LaneConst C;
-        C.q0 = P.q0[ j ]; C.mass0 = P.mass0[ j ]; C.beta0 = P.beta0[ j ];
-        C.p0c = P.p0c[ j ]; C.chi = P.chi[ j ]; C.charge_ratio = P.charge_ratio[ j ];
-        cavity( L, C, e );
-        return true;
-    }
-    case ST_OP_XYSHIFT: xyshift( L, e ); return true;
-    case ST_OP_SROTATION: srotation( L, e ); return true;
-    case ST_OP_LIMIT_RECT: return limit_rect_ok( L, e );
-    case ST_OP_LIMIT_ELLIPSE: return limit_ellipse_ok( L, e );
-    case ST_OP_LIMIT_RECT_ELLIPSE: return limit_rect_ellipse_ok( L, e );
-    case ST_OP_DIPEDGE: dipedge( L, e ); return true;
-    default: fail = true; return false;
-    }
-}
-
-// Record at byte offset ``off`` of the stream (32-bit offsets: one scalar
-// add per element and the offset form of the scalar load).
-ST_DEV cptr_t rec_at( cptr_t lat, uint32_t off )
-{
-    return ( cptr_t )( ( const __attribute__( ( address_space( 4 ) ) ) char* )lat + off );
-}
-
 ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t tid, const Args& a )
 {
-    // with a list, lane tid tracks particle list[tid] (the particles still
-    // to be tracked, gathered by st_k_select_active); else particle tid
-    bool const in_range = ( a.list != nullptr ) ? tid < a.n_list : tid < P.n;
-    int64_t const i = ( a.list != nullptr && in_range ) ? ( int64_t )a.list[ tid ] : tid;
-    int64_t const li = in_range ? i : 0;          // out-of-range lanes never store
-    Lane L;
-    L.x = P.x[ li ]; L.px = P.px[ li ]; L.y = P.y[ li ]; L.py = P.py[ li ];
-    L.zeta = P.zeta[ li ]; L.delta = P.delta[ li ]; L.rpp = P.rpp[ li ];
-    L.rvv = P.rvv[ li ]; L.psigma = P.psigma[ li ]; L.s = P.s[ li ];
-    double const chi = P.chi[ li ];
-    int64_t const start = P.at_element[ li ];
-    int64_t at_turn = P.at_turn[ li ];
-    bool const active0 = in_range && a.n_elem > 0 && a.until_turn > at_turn &&
-                         P.state[ li ] == 1;
-    // a launch may cover only part of the requested turns; the lane then
-    // stops at a turn boundary exactly as if until_turn were turn_stop
-    int64_t const turn_stop = ( a.max_turns > 0 && a.until_turn - at_turn > a.max_turns )
-                                  ? at_turn + a.max_turns : a.until_turn;
-    int32_t status = ( in_range && a.n_elem <= 0 ) ? -1 : 0;
-
-    // A lane leaves ``act`` once, after its final state has been stored.  A
-    // wave whose lanes are all lost finishes the turn without effect.
-    uint64_t act = ballot( active0 );
-    while( act != 0u )
-    {
-        uint32_t off = 0;
-        uint64_t h = rec_at( a.lat, 0 )[ 0 ];
-        for( ;; )
-        {
-#if defined( ST_FMA_BUILD )
-            double const rvvm1 = L.rvv - 1.0;         // changes only at a Cavity
-#endif
-            while( ( ( uint32_t )h & ST_OP_RARE_BIT ) == 0u )   // Drift / Multipole run
-            {
-                cptr_t const r = rec_at( a.lat, off );
-                uint32_t const noff = off + ST_REC_BYTES( h );
-                uint64_t const hn = rec_at( a.lat, noff )[ 0 ];
-                if( ( uint32_t )h & ST_OP_MP_BIT )
-                {
-                    multipole( L, chi, r + ST_REC_PAYLOAD, ST_REC_AUX( h ), L.delta );
-                }
-                else
-                {
-#if defined( ST_FMA_BUILD )
-                    drift_rvvm1( L, rd( r, ST_REC_PAYLOAD ), rvvm1 );
-#else
-                    drift( L, rd( r, ST_REC_PAYLOAD ) );
-#endif
-                    uint64_t const lost = limit_global_lost_mask( L ) & act;
-                    if( lost != 0u )
-                    {
-                        if( lane_in( lost ) )
-                            store_lane_state( P, i, L, 0, start + ri( r, 1 ), at_turn );
-                        act &= ~lost;
-                    }
-                }
-                off = noff;
-                h = hn;
-            }
-            unsigned const op = ST_REC_OP( h );
-            if( op == ST_OP_END ) break;
-            cptr_t const r = rec_at( a.lat, off );
-            uint32_t const noff = off + ST_REC_BYTES( h );
-            uint64_t const hn = rec_at( a.lat, noff )[ 0 ];
-            bool fail = false;
-            bool const ok = lean_other( op, r + ST_REC_PAYLOAD, L, P, li, fail );
-            uint64_t const lost = ballot( !ok ) & act;
-            if( lost != 0u )
-            {
-                if( lane_in( lost ) )
-                {
-                    store_lane_state( P, i, L, 0, start + ri( r, 1 ), at_turn );
-                    if( fail ) status = -1;
-                }
-                act &= ~lost;
-            }
-            off = noff;
-            h = hn;
-        }
-        if( lane_in( act ) ) { ++at_turn; L.s = 0.0; }
-        uint64_t const fin = ballot( at_turn >= turn_stop ) & act;
-        if( fin != 0u )
-        {
-            if( lane_in( fin ) ) store_lane_state( P, i, L, 1, start, at_turn );
-            act &= ~fin;
-        }
-    }
-    if( in_range && status != 0 ) atomicOr( a.status_out, status );
+    track_lean_lane( P, a.list, a.list != nullptr ? a.n_list : P.n, tid, a.lat, a.n_elem,
+                     a.until_turn, a.max_turns, a.status_out );
 }
 
 }  // namespace
 
-// Indices of the particles a track_until call has work for (state 1 and
-// at_turn < until_turn), appended wave by wave to ``list``: one atomic add
-// per wave reserves the wave's slots.  The order of the list is not the
-// particle order; every particle is tracked independently, so the results
-// do not depend on it.
-// count[1] receives the smallest at_turn among them (clamped to int32).
+// Gather of the particles a track_until call has work for (select_active_lane).
 __global__ void __launch_bounds__( 256 ) KN( st_k_select_active )( st_dev_particles_t P,
     int64_t until_turn, int32_t* list, int32_t* count )
 {
-    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t const t = ( i < P.n ) ? P.at_turn[ i ] : 0;
-    bool const want = i < P.n && P.state[ i ] == 1 && t < until_turn;
-    uint64_t const m = ballot( want );
-    if( m == 0u ) return;
-    int32_t const t32 = ( int32_t )( want ? ( t < 0 ? 0 : ( t > INT32_MAX ? INT32_MAX : t ) )
-                                          : INT32_MAX );
-    int32_t tmin = t32;
-    for( int off = 32; off > 0; off >>= 1 ) tmin = min( tmin, __shfl_xor( tmin, off ) );
-    int base = 0;
-    if( __lane_id() == 0 )
-    {
-        base = atomicAdd( &count[ 0 ], __popcll( m ) );
-        atomicMin( &count[ 1 ], tmin );
-    }
-    base = __shfl( base, 0 );
-    uint64_t const below = ( __lane_id() == 0 ) ? 0u : ( m << ( 64 - __lane_id() ) );
-    if( want ) list[ base + __popcll( below ) ] = ( int32_t )i;
+    select_active_lane( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, until_turn,
+                        list, count );
 }
 
 __global__ void __launch_bounds__( 256 ) KN( st_k_track_until )( st_dev_particles_t P, Args a )

@@ -147,6 +147,7 @@ def lhc_like_ring(seed=0, n_cells=400, tune=80.31, rel_err=1e-4, k2_scale=0.02,
     counts = {"drift": 0, "multipole": 0, "xyshift": 0, "srotation": 0,
               "ellipse": 0, "cavity": 0}
     orders = []
+    mp_design = []            # (object index, knl design, ksl design, error scale)
     misc_toggle = 0
     for c in range(n_cells):
         halves = []
@@ -179,6 +180,7 @@ def lhc_like_ring(seed=0, n_cells=400, tune=80.31, rel_err=1e-4, k2_scale=0.02,
                     nn = np.arange(order + 1)
                     scale = rel_err * k1 * np.array([math.factorial(int(m)) for m in nn]) \
                         * r0 ** (1.0 - nn)
+                    mp_design.append((eb.n_objects, knl.copy(), ksl.copy(), scale))
                     knl = knl + scale * erng.normal(0.0, 1.0, order + 1)
                     ksl = ksl + scale * erng.normal(0.0, 1.0, order + 1)
                     be.Multipole(cbuffer=eb, knl=knl, ksl=ksl)
@@ -197,8 +199,64 @@ def lhc_like_ring(seed=0, n_cells=400, tune=80.31, rel_err=1e-4, k2_scale=0.02,
         counts["ellipse"] += 1
     if return_info:
         return eb, {"counts": counts, "orders": orders, "k1": k1, "mu": mu,
-                    "half_cells": (l1, l2)}
+                    "half_cells": (l1, l2), "mp_design": mp_design}
     return eb
+
+
+def lhc_like_ring_family(n_seeds, seed=0, first_error_seed=0, **kw):
+    """``n_seeds`` machine seeds of one ring: the structure of
+    lhc_like_ring(seed) with the multipole-error tables of error_seed =
+    first_error_seed + k.  Ring k equals lhc_like_ring(seed, error_seed=
+    first_error_seed + k) word for word; the structure is generated once and
+    only the bal arrays are rewritten (same draws, same arithmetic)."""
+    base, info = lhc_like_ring(seed=seed, error_seed=first_error_seed, return_info=True, **kw)
+    mp = info["mp_design"]
+    # word offset of every multipole's bal array in the buffer
+    words = base.words
+    bal_off = []
+    for idx, knl, _, _ in mp:
+        o = base.get_object_word_offset(idx)
+        bal_off.append((int(words[o + 4]) - base.pointer) // 8)
+    inv_f = [np.array([be._inv_factorial(i) for i in range(len(knl))]) for _, knl, _, _ in mp]
+    rings = [base]
+    for k in range(1, n_seeds):
+        erng = np.random.default_rng(10_000 + int(first_error_seed + k))
+        cb = CBuffer.frombytes(base.tobytes())
+        w = cb.words.view(np.float64)
+        for (idx, knl, ksl, scale), off, inv in zip(mp, bal_off, inv_f):
+            n = len(knl)
+            nknl = knl + scale * erng.normal(0.0, 1.0, n)
+            nksl = ksl + scale * erng.normal(0.0, 1.0, n)
+            w[off:off + 2 * n:2] = nknl * inv
+            w[off + 1:off + 2 * n:2] = nksl * inv
+        rings.append(cb)
+    return rings
+
+
+def batch_workload(n_seeds=64, seed=0, min_particles=1000, max_particles=100000,
+                   count_seed=1, beta=236.0, max_sigma=20.0, delta=2.7e-4):
+    """Multi-seed workload: lhc_like_ring_family(n_seeds, seed) and, per
+    seed, U(min_particles, max_particles) particles (numpy
+    default_rng(count_seed).integers) on the first points of an
+    amplitude-major polar grid (64 angles, amplitudes up to ``max_sigma``
+    sigma of twiss_sigma(beta)).  Returns (lattices, particle buffers,
+    counts)."""
+    rings = lhc_like_ring_family(n_seeds, seed=seed)
+    counts = np.random.default_rng(count_seed).integers(min_particles, max_particles + 1,
+                                                        n_seeds)
+    sig = twiss_sigma(beta=beta)
+    parts = []
+    for n in counts:
+        n = int(n)
+        n_amp = (n + 63) // 64
+        r = np.linspace(max_sigma / n_amp, max_sigma, n_amp)
+        th = (np.arange(64) + 0.5) * (0.5 * np.pi / 64)
+        rr, tt = np.meshgrid(r, th, indexing="ij")
+        x = (rr * np.cos(tt) * sig).ravel()[:n]
+        y = (rr * np.sin(tt) * sig).ravel()[:n]
+        pb, _ = make_particles(n, x=x, y=y, delta=delta)
+        parts.append(pb)
+    return rings, parts, counts
 
 
 def flops_per_particle_turn(ebuf):

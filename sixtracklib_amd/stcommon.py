@@ -121,6 +121,22 @@ _fn("st_TrackJob_required_output_sets", ct.c_int64,
 _fn("st_TrackJob_last_kernel_time_ms", ct.c_double, [st_TrackJob_p])
 _fn("st_TrackJob_uses_fma", ct.c_int, [st_TrackJob_p])
 _fn("st_TrackJob_uses_full_kernel", ct.c_int, [st_TrackJob_p])
+
+# -- batched multi-seed tracking (include/sixtracklib_amd/track_batch.h) ---
+st_TrackBatch_p = ct.c_void_p
+_fn("st_TrackBatch_new", st_TrackBatch_p,
+    [ct.c_int64, ct.POINTER(st_Buffer_p), ct.POINTER(st_Buffer_p), ct.c_char_p])
+_fn("st_TrackBatch_delete", None, [st_TrackBatch_p])
+_fn("st_TrackBatch_track_until", st_track_status_t, [st_TrackBatch_p, ct.c_int64])
+_fn("st_TrackBatch_collect", ct.c_int, [st_TrackBatch_p])
+_fn("st_TrackBatch_push", ct.c_int, [st_TrackBatch_p])
+_fn("st_TrackBatch_set_config", ct.c_int, [st_TrackBatch_p, ct.c_char_p])
+_fn("st_TrackBatch_get_config", ct.c_int64, [st_TrackBatch_p, ct.c_char_p, ct.c_int64])
+_fn("st_TrackBatch_autotune", ct.c_int, [st_TrackBatch_p, ct.c_int64])
+_fn("st_TrackBatch_num_seeds", ct.c_int64, [st_TrackBatch_p])
+_fn("st_TrackBatch_total_particles", ct.c_int64, [st_TrackBatch_p])
+_fn("st_TrackBatch_last_kernel_time_ms", ct.c_double, [st_TrackBatch_p])
+
 _fn("st_Hip_num_devices", ct.c_int, [])
 _fn("st_Hip_last_error_str", ct.c_char_p, [])
 

@@ -153,3 +153,16 @@ def test_trackjob_without_device_raises():
     pb, _ = lattices.gaussian_beam(8)
     with pytest.raises(RuntimeError):
         TrackJob(eb, pb)
+
+
+def test_trackbatch_argument_and_device_errors():
+    from sixtracklib_amd import stcommon as st
+    from sixtracklib_amd.trackbatch import TrackBatch
+    with pytest.raises(ValueError):
+        TrackBatch([], [])
+    eb = lattices.fodo_ring(n_cells=2)
+    pb, _ = lattices.gaussian_beam(10)
+    if st.st_Hip_num_devices() > 0:
+        pytest.skip("a device is present: construction would succeed")
+    with pytest.raises(RuntimeError):
+        TrackBatch([eb], [pb])
