@@ -1,0 +1,329 @@
+// Persistent-kernel scheduling of lean until-turn tracking (host side),
+// shared by NS(TrackBatch) (many lattices) and NS(TrackJob) (one lattice,
+// one or more particle sets).  Library-internal C++.
+//
+// A "seed" is one (particle set, lean lattice) pair.  run(U, C, K) enqueues
+// C chunks, each
+//   reset   tile counters and per-seed [count, smallest at_turn];
+//   select  per seed, the particles with turns to go -> the seed's list;
+//   track   the persistent grid: waves take 64-particle tiles from their
+//           queue (seeds grouped per XCD with queues=8), track each for at
+//           most K turns, steal from other queues when theirs is drained;
+// optionally captured once into a HIP graph.  Everything is enqueued on
+// the caller's stream; nothing here synchronises except select_sync().
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cstdio>
+#include <cstdlib>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "batch.h"
+
+extern "C" void st_internal_set_error( char const* msg );
+
+namespace st_batch
+{
+inline bool hip_ok( hipError_t e, const char* what )
+{
+    if( e == hipSuccess ) return true;
+    std::string m = std::string( what ) + ": " + hipGetErrorString( e );
+    st_internal_set_error( m.c_str() );
+    return false;
+}
+
+inline bool fail( const char* what )
+{
+    st_internal_set_error( what );
+    return false;
+}
+
+struct Config
+{
+    bool fma = false;
+    int block = 256;
+    int waves_per_cu = 32;
+    int queues = 8;
+    int64_t turns_per_launch = 0;
+    bool graph = true;
+
+    std::string str() const
+    {
+        char b[ 160 ];
+        std::snprintf( b, sizeof( b ), "fma=%d block=%d waves_per_cu=%d queues=%d "
+                       "turns_per_launch=%lld graph=%d", fma ? 1 : 0, block, waves_per_cu,
+                       queues, ( long long )turns_per_launch, graph ? 1 : 0 );
+        return b;
+    }
+
+    // one "key=value" token; unknown keys are ignored (false only on a bad value)
+    bool parse_token( std::string const& k, std::string const& v )
+    {
+        long const iv = std::strtol( v.c_str(), nullptr, 10 );
+        if( k == "fma" ) fma = iv != 0;
+        else if( k == "block" )
+        {
+            if( iv != 64 && iv != 128 && iv != 256 ) return fail( "block must be 64, 128 or 256" );
+            block = ( int )iv;
+        }
+        else if( k == "waves_per_cu" )
+        {
+            if( iv < 1 || iv > 32 ) return fail( "waves_per_cu must be 1..32" );
+            waves_per_cu = ( int )iv;
+        }
+        else if( k == "queues" )
+        {
+            if( iv != 1 && iv != 8 ) return fail( "queues must be 1 or 8" );
+            queues = ( int )iv;
+        }
+        else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
+        else if( k == "graph" ) graph = iv != 0;
+        return true;
+    }
+
+    bool parse( const char* cfg )
+    {
+        if( cfg == nullptr ) return true;
+        std::string s = cfg;
+        for( char& ch : s ) if( ch == ',' || ch == ';' ) ch = ' ';
+        size_t pos = 0;
+        while( pos < s.size() )
+        {
+            size_t const end = s.find( ' ', pos );
+            std::string const tok = s.substr( pos, end == std::string::npos ? std::string::npos : end - pos );
+            pos = ( end == std::string::npos ) ? s.size() : end + 1;
+            size_t const eq = tok.find( '=' );
+            if( eq == std::string::npos ) continue;
+            if( !parse_token( tok.substr( 0, eq ), tok.substr( eq + 1 ) ) ) return false;
+        }
+        return true;
+    }
+};
+
+struct BatchRunner
+{
+    Config cfg;
+    int n_cu = 0;
+    std::vector< int64_t > n_part, n_elem;
+    int64_t total_particles = 0, max_particles = 0;
+
+    st_batch_seed_t* d_seeds = nullptr;
+    int32_t* d_lists = nullptr;
+    int32_t* d_counts = nullptr;       // [2 * S]
+    int32_t* d_qseeds = nullptr;
+    int32_t* d_qbegin = nullptr;
+    int32_t* d_qnext = nullptr;        // [8]
+    int32_t* h_counts = nullptr;       // pinned [2 * S]
+    hipGraphExec_t gexec = nullptr;
+    std::string gkey;
+
+    BatchRunner() = default;
+    BatchRunner( const BatchRunner& ) = delete;
+    BatchRunner& operator=( const BatchRunner& ) = delete;
+    ~BatchRunner() { release(); }
+
+    void release()
+    {
+        if( gexec ) ( void )hipGraphExecDestroy( gexec );
+        gexec = nullptr; gkey.clear();
+        for( void* p : { ( void* )d_seeds, ( void* )d_lists, ( void* )d_counts, ( void* )d_qseeds,
+                         ( void* )d_qbegin, ( void* )d_qnext } )
+            if( p ) ( void )hipFree( p );
+        if( h_counts ) ( void )hipHostFree( h_counts );
+        d_seeds = nullptr; d_lists = d_counts = d_qseeds = d_qbegin = d_qnext = h_counts = nullptr;
+        n_part.clear(); n_elem.clear(); total_particles = max_particles = 0;
+    }
+
+    int64_t n_seeds() const { return ( int64_t )n_part.size(); }
+
+    // seeds: particle views, lattices and n_elem set; their list fields are
+    // assigned here (one list region per seed)
+    bool init( std::vector< st_batch_seed_t > seeds, hipStream_t stream )
+    {
+        release();
+        int dev = 0;
+        if( !hip_ok( hipGetDevice( &dev ), "hipGetDevice" ) ||
+            !hip_ok( hipDeviceGetAttribute( &n_cu, hipDeviceAttributeMultiprocessorCount, dev ),
+                     "hipDeviceGetAttribute" ) ) return false;
+        int64_t const S = ( int64_t )seeds.size();
+        for( const st_batch_seed_t& s : seeds )
+        {
+            if( s.P.n < 0 || s.P.n >= ( int64_t )INT32_MAX ) return fail( "particle set too large" );
+            n_part.push_back( s.P.n );
+            n_elem.push_back( s.n_elem );
+            total_particles += s.P.n;
+            max_particles = std::max( max_particles, s.P.n );
+        }
+        if( !hip_ok( hipMalloc( &d_lists, std::max< int64_t >( total_particles, 1 ) * sizeof( int32_t ) ),
+                     "hipMalloc lists" ) ||
+            !hip_ok( hipMalloc( &d_seeds, std::max< int64_t >( S, 1 ) * sizeof( st_batch_seed_t ) ),
+                     "hipMalloc seeds" ) ||
+            !hip_ok( hipMalloc( &d_counts, std::max< int64_t >( S, 1 ) * 2 * sizeof( int32_t ) ),
+                     "hipMalloc counts" ) ||
+            !hip_ok( hipHostMalloc( &h_counts, std::max< int64_t >( S, 1 ) * 2 * sizeof( int32_t ) ),
+                     "hipHostMalloc counts" ) ||
+            !hip_ok( hipMalloc( &d_qnext, 8 * sizeof( int32_t ) ), "hipMalloc queues" ) ) return false;
+        int64_t list_off = 0;
+        for( int64_t s = 0; s < S; ++s )
+        {
+            seeds[ s ].list = d_lists + list_off;
+            list_off += n_part[ s ];
+        }
+        if( S > 0 && !hip_ok( hipMemcpyAsync( d_seeds, seeds.data(), S * sizeof( st_batch_seed_t ),
+                                              hipMemcpyHostToDevice, stream ), "upload seeds" ) )
+            return false;
+        return build_queues( stream );
+    }
+
+    // Seeds -> queues.  queues=8: greedy balance of particle counts over 8
+    // queues (largest seed first), one queue per XCD: blocks are handed to
+    // the XCDs round robin, so block b runs on XCD b % 8 and starts on queue
+    // b % 8.  queues=1: all seeds in one queue, largest first.
+    bool build_queues( hipStream_t stream )
+    {
+        if( gexec ) { ( void )hipGraphExecDestroy( gexec ); gexec = nullptr; gkey.clear(); }
+        int64_t const S = n_seeds();
+        int const nq = cfg.queues;
+        std::vector< int64_t > order( S );
+        std::iota( order.begin(), order.end(), 0 );
+        std::stable_sort( order.begin(), order.end(),
+                          [ & ]( int64_t a, int64_t b ) { return n_part[ a ] > n_part[ b ]; } );
+        std::vector< std::vector< int32_t > > q( nq );
+        std::vector< int64_t > load( nq, 0 );
+        for( int64_t s : order )
+        {
+            int const k = ( int )( std::min_element( load.begin(), load.end() ) - load.begin() );
+            q[ k ].push_back( ( int32_t )s );
+            load[ k ] += n_part[ s ];
+        }
+        std::vector< int32_t > qs, qb( 1, 0 );
+        for( int k = 0; k < nq; ++k )
+        {
+            qs.insert( qs.end(), q[ k ].begin(), q[ k ].end() );
+            qb.push_back( ( int32_t )qs.size() );
+        }
+        if( d_qseeds ) { ( void )hipFree( d_qseeds ); d_qseeds = nullptr; }
+        if( d_qbegin ) { ( void )hipFree( d_qbegin ); d_qbegin = nullptr; }
+        if( !hip_ok( hipMalloc( &d_qseeds, std::max< size_t >( qs.size(), 1 ) * sizeof( int32_t ) ),
+                     "hipMalloc queue seeds" ) ||
+            !hip_ok( hipMalloc( &d_qbegin, qb.size() * sizeof( int32_t ) ), "hipMalloc queue begin" ) )
+            return false;
+        if( !qs.empty() &&
+            !hip_ok( hipMemcpyAsync( d_qseeds, qs.data(), qs.size() * sizeof( int32_t ),
+                                     hipMemcpyHostToDevice, stream ), "upload queues" ) ) return false;
+        return hip_ok( hipMemcpyAsync( d_qbegin, qb.data(), qb.size() * sizeof( int32_t ),
+                                       hipMemcpyHostToDevice, stream ), "upload queues" ) &&
+               hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" );
+    }
+
+    st_batch_sched_t sched() const
+    {
+        st_batch_sched_t s{};
+        s.seeds = d_seeds; s.counts = d_counts; s.queue_seeds = d_qseeds;
+        s.queue_begin = d_qbegin; s.queue_next = d_qnext;
+        s.n_seeds = ( int32_t )n_seeds(); s.n_queues = cfg.queues;
+        return s;
+    }
+
+    // turns per chunk: about 5e11 particle-element passes per launch (25
+    // turns of 1M particles through 20k elements), so that a launch's
+    // start-up and drain stay negligible while particles lost during a
+    // chunk stop costing lanes at the next
+    int64_t chunk_turns( int64_t n_active ) const
+    {
+        if( cfg.turns_per_launch > 0 ) return cfg.turns_per_launch;
+        int64_t ne = 1;
+        for( int64_t e : n_elem ) ne = std::max( ne, e );
+        double const work = ( double )std::max< int64_t >( n_active, 1 ) * ( double )ne;
+        double const k = 5.0e11 / work;
+        return k >= 1.0e9 ? ( int64_t )1000000000 : ( k < 1.0 ? 1 : ( int64_t )k );
+    }
+
+    int grid() const
+    {
+        int const waves_per_block = cfg.block / 64;
+        int const g = ( n_cu > 0 ? n_cu : 256 ) * cfg.waves_per_cu / waves_per_block;
+        return g < cfg.queues ? cfg.queues : g;
+    }
+
+    bool enqueue_reset_select( int64_t until_turn, hipStream_t stream ) const
+    {
+        st_batch_sched_t const s = sched();
+        return hip_ok( cfg.fma ? st_launch_batch_reset_fma( &s, stream )
+                               : st_launch_batch_reset_exact( &s, stream ), "launch batch reset" ) &&
+               hip_ok( cfg.fma ? st_launch_batch_select_fma( &s, max_particles, until_turn, stream )
+                               : st_launch_batch_select_exact( &s, max_particles, until_turn, stream ),
+                       "launch batch select" );
+    }
+
+    bool enqueue_chunks( int64_t until_turn, int64_t chunks, int64_t k, int32_t* d_status,
+                         hipStream_t stream ) const
+    {
+        st_batch_sched_t const s = sched();
+        for( int64_t c = 0; c < chunks; ++c )
+        {
+            if( !enqueue_reset_select( until_turn, stream ) ) return false;
+            hipError_t const e = cfg.fma
+                ? st_launch_batch_track_fma( &s, until_turn, k, d_status, cfg.block, grid(), stream )
+                : st_launch_batch_track_exact( &s, until_turn, k, d_status, cfg.block, grid(), stream );
+            if( !hip_ok( e, "launch batch track" ) ) return false;
+        }
+        return true;
+    }
+
+    // one select pass, read back: total particles with turns to go and the
+    // smallest at_turn among them (synchronises the stream)
+    bool select_sync( int64_t until_turn, hipStream_t stream, int64_t& n_active, int64_t& min_turn )
+    {
+        int64_t const S = n_seeds();
+        n_active = 0;
+        min_turn = INT64_MAX;
+        if( S == 0 ) { min_turn = 0; return true; }
+        if( !enqueue_reset_select( until_turn, stream ) ||
+            !hip_ok( hipMemcpyAsync( h_counts, d_counts, S * 2 * sizeof( int32_t ),
+                                     hipMemcpyDeviceToHost, stream ), "counts D2H" ) ||
+            !hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" ) ) return false;
+        for( int64_t s = 0; s < S; ++s )
+        {
+            n_active += h_counts[ 2 * s ];
+            if( h_counts[ 2 * s ] > 0 ) min_turn = std::min< int64_t >( min_turn, h_counts[ 2 * s + 1 ] );
+        }
+        if( n_active == 0 ) min_turn = until_turn;
+        return true;
+    }
+
+    // C chunks of K turns towards until_turn; the OR of statuses into
+    // *d_status.  Captured into (and replayed from) a HIP graph with
+    // cfg.graph, else launched directly.
+    bool run( int64_t until_turn, int64_t chunks, int64_t k, int32_t* d_status, hipStream_t stream )
+    {
+        if( chunks <= 0 ) return true;
+        if( !cfg.graph ) return enqueue_chunks( until_turn, chunks, k, d_status, stream );
+        char kb[ 96 ];
+        std::snprintf( kb, sizeof( kb ), " until=%lld chunks=%lld k=%lld status=%p",
+                       ( long long )until_turn, ( long long )chunks, ( long long )k, ( void* )d_status );
+        std::string const key = cfg.str() + kb;
+        bool ok = true;
+        if( gexec == nullptr || key != gkey )
+        {
+            if( gexec ) { ( void )hipGraphExecDestroy( gexec ); gexec = nullptr; gkey.clear(); }
+            hipGraph_t g = nullptr;
+            ok = hip_ok( hipStreamBeginCapture( stream, hipStreamCaptureModeThreadLocal ),
+                         "hipStreamBeginCapture" );
+            bool const enq = ok && enqueue_chunks( until_turn, chunks, k, d_status, stream );
+            hipError_t const ec = ok ? hipStreamEndCapture( stream, &g ) : hipSuccess;
+            ok = ok && enq && hip_ok( ec, "hipStreamEndCapture" ) &&
+                 hip_ok( hipGraphInstantiate( &gexec, g, nullptr, nullptr, 0 ), "hipGraphInstantiate" );
+            if( g ) ( void )hipGraphDestroy( g );
+            if( ok ) gkey = key;
+            else if( gexec ) { ( void )hipGraphExecDestroy( gexec ); gexec = nullptr; }
+        }
+        return ok && hip_ok( hipGraphLaunch( gexec, stream ), "hipGraphLaunch" );
+    }
+};
+}  // namespace st_batch

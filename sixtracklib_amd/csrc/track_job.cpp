@@ -28,6 +28,8 @@
 #include "sixtracklib_amd/lattice_pack.h"
 #include "lattice.h"
 #include "track_launch.h"
+#include "batch.h"
+#include "batch_runner.h"
 
 namespace
 {
@@ -143,10 +145,11 @@ struct NS(TrackJobBase)
     std::vector< uint32_t > rec_offsets, monitor_fields;
     std::vector< uint64_t > lat_host;
     bool lat_full = false;
-    int32_t* d_status = nullptr; int32_t* h_status = nullptr;   // [status, count, min turn]
-    int32_t* d_list = nullptr; int64_t list_cap = 0;           // active-particle list
-    bool compact = true;
-    int64_t turns_per_launch = 0;                              // 0: from the work size
+    int32_t* d_status = nullptr; int32_t* h_status = nullptr;
+    bool compact = true;                 // lean lattices: persistent tiles over active particles
+    int64_t turns_per_launch = 0;        // 0: from the work size
+    st_batch::BatchRunner runner;        // scheduling of the lean until path
+    bool runner_ready = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     float last_kernel_ms = 0.0f;
     std::vector< st_dev_particles_t > views;
@@ -160,8 +163,8 @@ struct NS(TrackJobBase)
         if( d_obuf ) ( void )hipFree( d_obuf );
         if( d_lat ) ( void )hipFree( d_lat );
         if( d_status ) ( void )hipFree( d_status );
-        if( d_list ) ( void )hipFree( d_list );
-        d_list = nullptr; list_cap = 0;
+        runner.release();
+        runner_ready = false;
         if( h_status ) ( void )hipHostFree( h_status );
         if( ev_start ) ( void )hipEventDestroy( ev_start );
         if( ev_stop ) ( void )hipEventDestroy( ev_stop );
@@ -213,6 +216,7 @@ struct NS(TrackJobBase)
 
     bool pack_lattice()
     {
+        runner_ready = false;                   // seeds point at the lattice
         int64_t n_elem_l = 0, n_mon = 0;
         const unsigned char* eb = NS(Buffer_get_const_data_begin)( ebuf );
         int64_t const nw = NS(Lattice_pack)( eb, nullptr, 0, nullptr, &n_elem_l, nullptr, &n_mon );
@@ -332,8 +336,8 @@ struct NS(TrackJobBase)
         if( !hip_ok( hipStreamCreate( &stream ), "hipStreamCreate" ) ) return false;
         if( !hip_ok( hipEventCreate( &ev_start ), "hipEventCreate" ) ||
             !hip_ok( hipEventCreate( &ev_stop ), "hipEventCreate" ) ) return false;
-        if( !hip_ok( hipMalloc( &d_status, 4 * sizeof( int32_t ) ), "hipMalloc status" ) ) return false;
-        if( !hip_ok( hipHostMalloc( &h_status, 4 * sizeof( int32_t ) ), "hipHostMalloc" ) ) return false;
+        if( !hip_ok( hipMalloc( &d_status, sizeof( int32_t ) ), "hipMalloc status" ) ) return false;
+        if( !hip_ok( hipHostMalloc( &h_status, sizeof( int32_t ) ), "hipHostMalloc" ) ) return false;
         pbuf_bytes = NS(Buffer_get_size)( pbuf );
         if( !hip_ok( hipMalloc( &d_pbuf, pbuf_bytes ), "hipMalloc particles" ) ) return false;
         if( !upload_image( pbuf, d_pbuf, pbuf_bytes ) ) return false;
@@ -372,81 +376,55 @@ struct NS(TrackJobBase)
         return last_status;
     }
 
-    bool ensure_list( int64_t n )
+    // Lean lattices (the default): the persistent scheduling of the batched
+    // form with the job's particle sets as seeds (batch_runner.h).  Only
+    // the particles with turns to go take lanes; a long request is split
+    // into chunks of a few turns (list rebuilt between them, on the
+    // device), so lanes of particles lost during a chunk are reused at the
+    // next.  Every particle is tracked turn by turn on its own, so the
+    // results are those of one launch.  compact=0 (or a full lattice) runs
+    // one launch with one lane per particle.
+    bool ensure_runner()
     {
-        if( n <= list_cap ) return true;
-        if( d_list ) { ( void )hipFree( d_list ); d_list = nullptr; list_cap = 0; }
-        if( !hip_ok( hipMalloc( &d_list, sizeof( int32_t ) * ( size_t )n ), "hipMalloc list" ) )
-            return false;
-        list_cap = n;
-        return true;
-    }
-
-    // Lean lattices: the particles that still have turns to go (state 1,
-    // at_turn < until_turn) are gathered into an index list before each
-    // launch, so lost and finished particles occupy no lanes.  A long
-    // request is split into launches of about 5e11 particle-element passes
-    // each (25 turns of 1M particles through 20k elements, ~0.3 s) and the
-    // list is rebuilt between them: particles lost during a launch stop
-    // costing lanes at the next one, while each launch stays long enough
-    // for its start-up and drain to be negligible.  Every particle is tracked turn by turn on
-    // its own, so the results are those of one launch.
-    int64_t chunk_turns( int64_t n_active ) const
-    {
-        if( turns_per_launch > 0 ) return turns_per_launch;
-        double const work = ( double )n_active * ( double )( n_elem > 0 ? n_elem : 1 );
-        double const k = 5.0e11 / ( work > 1.0 ? work : 1.0 );
-        return k >= 1.0e9 ? ( int64_t )1000000000 : ( k < 1.0 ? 1 : ( int64_t )k );
-    }
-
-    bool select_active( const st_dev_particles_t& v, int64_t until_turn, int64_t& n_list,
-                        int64_t& min_turn )
-    {
-        static const int32_t init[ 2 ] = { 0, INT32_MAX };
-        bool const ok = ensure_list( v.n ) &&
-            hip_ok( hipMemcpyAsync( d_status + 1, init, sizeof( init ), hipMemcpyHostToDevice,
-                                    stream ), "select init" ) &&
-            hip_ok( fma ? st_launch_select_active_fma( &v, until_turn, d_list, d_status + 1, stream )
-                        : st_launch_select_active_exact( &v, until_turn, d_list, d_status + 1, stream ),
-                    "launch select_active" ) &&
-            hip_ok( hipMemcpyAsync( h_status + 1, d_status + 1, 2 * sizeof( int32_t ),
-                                    hipMemcpyDeviceToHost, stream ), "count D2H" ) &&
-            hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" );
-        n_list = ok ? h_status[ 1 ] : 0;
-        min_turn = ok ? h_status[ 2 ] : 0;
-        return ok;
+        if( runner_ready ) return true;
+        runner.cfg = st_batch::Config();
+        runner.cfg.fma = fma;
+        runner.cfg.block = ( block == 64 || block == 128 ) ? block : 256;
+        runner.cfg.turns_per_launch = turns_per_launch;
+        runner.cfg.graph = false;
+        std::vector< st_batch_seed_t > seeds;
+        for( const st_dev_particles_t& v : views )
+        {
+            st_batch_seed_t sd{};
+            sd.P = v; sd.lat = d_lat; sd.n_elem = n_elem; sd.list = nullptr;
+            seeds.push_back( sd );
+        }
+        runner_ready = runner.init( seeds, stream );
+        return runner_ready;
     }
 
     int32_t track_until( int64_t until_turn )
     {
         if( !begin_status() ) return end_status( false );
         bool ok = true;
+        if( compact && !lat_full && n_elem > 0 )
+        {
+            int64_t n_active = 0, min_turn = 0;
+            ok = ensure_runner() && runner.select_sync( until_turn, stream, n_active, min_turn );
+            if( ok && n_active > 0 )
+            {
+                int64_t const k = runner.chunk_turns( n_active );
+                int64_t const chunks = ( until_turn - min_turn + k - 1 ) / k;
+                ok = runner.run( until_turn, chunks, k, d_status, stream );
+            }
+            return end_status( ok );
+        }
         for( const st_dev_particles_t& v : views )
         {
-            if( !( compact && !lat_full && n_elem > 0 && v.n > 0 && v.n < ( int64_t )INT32_MAX ) )
-            {
-                hipError_t const err = fma
-                    ? st_launch_track_until_fma( &v, d_lat, n_elem, until_turn, lat_full, nullptr, 0,
-                                                 0, d_status, block, stream )
-                    : st_launch_track_until_exact( &v, d_lat, n_elem, until_turn, lat_full, nullptr,
-                                                   0, 0, d_status, block, stream );
-                ok = ok && hip_ok( err, "launch track_until" );
-                continue;
-            }
-            int64_t n_list = 0, min_turn = 0;
-            ok = ok && select_active( v, until_turn, n_list, min_turn );
-            while( ok && n_list > 0 )
-            {
-                int64_t const k = chunk_turns( n_list );
-                hipError_t const err = fma
-                    ? st_launch_track_until_fma( &v, d_lat, n_elem, until_turn, 0, d_list, n_list,
-                                                 k, d_status, block, stream )
-                    : st_launch_track_until_exact( &v, d_lat, n_elem, until_turn, 0, d_list, n_list,
-                                                   k, d_status, block, stream );
-                ok = hip_ok( err, "launch track_until" );
-                if( !ok || until_turn - min_turn <= k ) break;     // every lane reached until_turn
-                ok = select_active( v, until_turn, n_list, min_turn );
-            }
+            hipError_t const err = fma
+                ? st_launch_track_until_fma( &v, d_lat, n_elem, until_turn, lat_full, d_status, block, stream )
+                : st_launch_track_until_exact( &v, d_lat, n_elem, until_turn, lat_full, d_status, block, stream );
+            ok = ok && hip_ok( err, "launch track_until" );
         }
         return end_status( ok );
     }

@@ -197,9 +197,6 @@ struct Args
     st_ebe_config_t cfg;     // ebe mode
     st_dev_particles_t O;    // ebe output set
     int32_t* status_out;
-    const int32_t* list;     // lean until: particle indices to track (or null)
-    int64_t n_list;
-    int64_t max_turns;       // lean until: turns per lane in this launch (<= 0: all)
 };
 
 // Per-particle loops: track/track.h:610-670 (until), 672-777 (element by
@@ -304,19 +301,10 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
 
 ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t tid, const Args& a )
 {
-    track_lean_lane( P, a.list, a.list != nullptr ? a.n_list : P.n, tid, a.lat, a.n_elem,
-                     a.until_turn, a.max_turns, a.status_out );
+    track_lean_lane( P, nullptr, P.n, tid, a.lat, a.n_elem, a.until_turn, 0, a.status_out );
 }
 
 }  // namespace
-
-// Gather of the particles a track_until call has work for (select_active_lane).
-__global__ void __launch_bounds__( 256 ) KN( st_k_select_active )( st_dev_particles_t P,
-    int64_t until_turn, int32_t* list, int32_t* count )
-{
-    select_active_lane( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, until_turn,
-                        list, count );
-}
 
 __global__ void __launch_bounds__( 256 ) KN( st_k_track_until )( st_dev_particles_t P, Args a )
 {
@@ -357,19 +345,9 @@ static inline int sane_block( int block )
     return ( block <= 0 || block > 256 || ( block % 64 ) != 0 ) ? 256 : block;
 }
 
-extern "C" hipError_t KN( st_launch_select_active )( const st_dev_particles_t* P,
-    int64_t until_turn, int32_t* list, int32_t* count, hipStream_t stream )
-{
-    if( P->n <= 0 ) return hipSuccess;
-    hipLaunchKernelGGL( KN( st_k_select_active ), dim3( grid_for( P->n, 256 ) ), dim3( 256 ),
-                        0, stream, *P, until_turn, list, count );
-    return hipGetLastError();
-}
-
 extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
     const uint64_t* lat, int64_t n_elem, int64_t until_turn, int full,
-    const int32_t* list, int64_t n_list, int64_t max_turns, int32_t* status, int block,
-    hipStream_t stream )
+    int32_t* status, int block, hipStream_t stream )
 {
     if( P->n <= 0 ) return hipSuccess;
     block = sane_block( block );
@@ -383,10 +361,7 @@ extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
     }
     else
     {
-        int64_t const n = ( list != nullptr ) ? n_list : P->n;
-        if( list != nullptr && ( n_list <= 0 || n_list > P->n ) ) return hipSuccess;
-        a.list = list; a.n_list = n_list; a.max_turns = max_turns;
-        hipLaunchKernelGGL( KN( st_k_track_until_lean ), dim3( grid_for( n, block ) ),
+        hipLaunchKernelGGL( KN( st_k_track_until_lean ), dim3( grid_for( P->n, block ) ),
                             dim3( block ), 0, stream, *P, a );
     }
     return hipGetLastError();

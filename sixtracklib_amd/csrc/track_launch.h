@@ -8,12 +8,9 @@
 #include "lattice.h"
 
 #define ST_DECLARE_LAUNCHERS( SUF )                                              \
-    extern "C" hipError_t st_launch_select_active##SUF( const st_dev_particles_t* P, \
-        int64_t until_turn, int32_t* list, int32_t* count, hipStream_t stream );  \
     extern "C" hipError_t st_launch_track_until##SUF( const st_dev_particles_t* P, \
         const uint64_t* lat, int64_t n_elem, int64_t until_turn, int full,       \
-        const int32_t* list, int64_t n_list, int64_t max_turns, int32_t* status,  \
-        int block, hipStream_t stream );                                          \
+        int32_t* status, int block, hipStream_t stream );                         \
     extern "C" hipError_t st_launch_track_ebe##SUF( const st_dev_particles_t* P,  \
         const uint64_t* lat, int64_t n_elem, int64_t until_turn,                  \
         const st_ebe_config_t* cfg, const st_dev_particles_t* O, int32_t* status, \
