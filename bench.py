@@ -85,6 +85,53 @@ def cpu_baseline(eb, pb, budget_s=10.0, sample=8192, turns_per_chunk=2):
                                    "x %d turns of the same ring, %.1f s" % (len(idx), n, turn, dt))
 
 
+def batched_vs_sequential(n_seeds, turns):
+    """Batched form: n_seeds machine seeds of the 20k-element ring (own
+    multipole-error table each) with U(1000, 100000) particles per seed,
+    tracked ``turns`` turns by one TrackBatch (autotuned, chunks in one HIP
+    graph) and by n_seeds plain TrackJobs one after the other.  Both count
+    tracked particle-turns over the wall time of their track_until calls
+    (lattices and particles already on the device)."""
+    from sixtracklib_amd import lattices
+    from sixtracklib_amd.cobjects import CBuffer
+    from sixtracklib_amd.particles import particles_in
+    from sixtracklib_amd.trackjob import TrackJob
+    from sixtracklib_amd.trackbatch import TrackBatch
+    rings, parts, counts = lattices.batch_workload(n_seeds)
+    bparts = [CBuffer.frombytes(p.tobytes()) for p in parts]
+
+    def tracked(bufs):
+        return sum(int(np.sum(np.array(particles_in(b).at_turn))) for b in bufs)
+
+    jobs = [TrackJob(eb, pb, config_str="fma=1") for eb, pb in zip(rings, parts)]
+    t0 = time.perf_counter()
+    for j in jobs:
+        j.track_until(turns)
+    seq_wall = time.perf_counter() - t0
+    for j in jobs:
+        j.collect_particles()
+    del jobs
+    batch = TrackBatch(rings, bparts, config_str="fma=1")
+    cfg = batch.autotune(trial_turns=2)
+    t0 = time.perf_counter()
+    batch.track_until(turns)
+    b_wall = time.perf_counter() - t0
+    b_kernel_ms = batch.last_kernel_time_ms
+    batch.collect()
+    del batch
+    seq_t, b_t = tracked(parts), tracked(bparts)
+    fields = ("x", "px", "y", "py", "zeta", "delta", "s", "at_element", "at_turn", "state")
+    same = all(np.array_equal(np.asarray(getattr(particles_in(a), f)),
+                              np.asarray(getattr(particles_in(b), f)))
+               for a, b in zip(parts, bparts) for f in fields)
+    return {"seeds": n_seeds, "particles": int(np.sum(counts)), "turns": turns,
+            "value": b_t / b_wall, "unit": "particle-turns/s",
+            "sequential_value": seq_t / seq_wall, "speedup": seq_wall / b_wall,
+            "kernel_seconds": b_kernel_ms / 1e3, "tracked_particle_turns": b_t,
+            "results_identical_to_sequential": bool(same and seq_t == b_t),
+            "autotuned_config": cfg}
+
+
 def calibration_copy(nbytes):
     """One device-to-device copy of ``nbytes`` through the HIP runtime (its
     blit kernel reads and writes exactly nbytes of HBM)."""
@@ -123,6 +170,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--exact-steps", type=int, default=1,
                     help="extra timed steps with the exact (no-FMA) build, reported beside")
+    ap.add_argument("--batch-seeds", type=int, default=64,
+                    help="batched form: machine seeds (0 disables the batched measurement)")
+    ap.add_argument("--batch-turns", type=int, default=100,
+                    help="batched form: turns tracked by the batch and by the sequential jobs")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device hipMemcpy of this many MiB (a copy "
                          "kernel of known traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -184,6 +235,10 @@ def main():
                  "other_build_value": tr_e / wall_e,
                  "other_build_tflops": tr_e * flops_pt / (kms_e / 1e3) / 1e12}
 
+    batched = None
+    if args.batch_seeds > 0:
+        batched = batched_vs_sequential(args.batch_seeds, args.batch_turns)
+
     cpu = None
     if not args.no_cpu_baseline:
         v, threads, desc = cpu_baseline(eb, pb, budget_s=args.cpu_budget)
@@ -219,6 +274,7 @@ def main():
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
                      "algorithmic_hbm_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     out.update(extra)
     print(json.dumps(out))

@@ -233,6 +233,59 @@ def lhc_like_ring_family(n_seeds, seed=0, first_error_seed=0, **kw):
     return rings
 
 
+def adjoint_sublattice(seed=0, n_cells=20, tune=4.31, rel_err=1e-4, k2_scale=0.02,
+                       r0=0.017, voltage=2e6, frequency=400e6, mean_drift=5.05,
+                       cbuffer=None):
+    """1,000-element sub-lattice for differentiable tracking: Drift,
+    Multipole of order 1-3 and one Cavity.
+
+    Each 50-element cell is [QF | 12 drifts, 12 multipoles (shuffled) | QD |
+    12 drifts, 12 multipoles (shuffled)], drift lengths U(0.1, 10) m
+    rescaled so each half cell is 12 x ``mean_drift`` long (identical linear
+    optics in every cell, thin-lens FODO phase advance 2 pi tune / n_cells).
+    Multipoles have a random order 1..3; half of those of order >= 2 carry a
+    design sextupole +-``k2_scale``; all get N(0, ``rel_err``) relative
+    errors on every coefficient (normalised to the quad strength at radius
+    ``r0``).  Cell 0 has a cavity in place of its first multipole."""
+    rng = np.random.default_rng(seed)
+    erng = np.random.default_rng(20_000 + int(seed))
+    eb = cbuffer if cbuffer is not None else CBuffer(
+        max_slots=n_cells * 50 * 16, max_objects=n_cells * 50 + 16, max_pointers=n_cells * 50)
+    half = 12 * mean_drift
+    mu = 2.0 * math.pi * tune / n_cells
+    k1 = 2.0 * math.sin(mu / 2.0) / half
+    for c in range(n_cells):
+        cav_done = c != 0
+        for quad in (+k1, -k1):
+            lengths = rng.uniform(0.1, 10.0, 12)
+            lengths *= half / lengths.sum()
+            lengths = iter(lengths)
+            slots = ["drift"] * 12 + ["mp"] * 12
+            rng.shuffle(slots)
+            for s in ["quad"] + slots:
+                if s == "drift":
+                    be.Drift(cbuffer=eb, length=float(next(lengths)))
+                    continue
+                if s == "mp" and not cav_done:
+                    be.Cavity(cbuffer=eb, voltage=voltage, frequency=frequency, lag=180.0)
+                    cav_done = True
+                    continue
+                order = 1 if s == "quad" else int(rng.integers(1, 4))
+                knl = np.zeros(order + 1)
+                ksl = np.zeros(order + 1)
+                if s == "quad":
+                    knl[1] = quad
+                elif order >= 2 and rng.random() < 0.5:
+                    knl[2] = k2_scale * (1.0 if rng.random() < 0.5 else -1.0)
+                nn = np.arange(order + 1)
+                scale = rel_err * k1 * np.array([math.factorial(int(m)) for m in nn]) \
+                    * r0 ** (1.0 - nn)
+                knl = knl + scale * erng.normal(0.0, 1.0, order + 1)
+                ksl = ksl + scale * erng.normal(0.0, 1.0, order + 1)
+                be.Multipole(cbuffer=eb, knl=knl, ksl=ksl)
+    return eb
+
+
 def batch_workload(n_seeds=64, seed=0, min_particles=1000, max_particles=100000,
                    count_seed=1, beta=236.0, max_sigma=20.0, delta=2.7e-4):
     """Multi-seed workload: lhc_like_ring_family(n_seeds, seed) and, per

@@ -44,14 +44,15 @@ class TrackBatch:
         self._c_buffers.append(p)
         return p
 
-    def _free_buffers(self):
+    def _free_buffers(self, _buffer_delete=st.st_Buffer_delete):
         for p in self._c_buffers:
-            st.st_Buffer_delete(p)
+            _buffer_delete(p)
         self._c_buffers = []
 
-    def __del__(self):
+    # library functions bound as defaults: module globals may be gone at exit
+    def __del__(self, _batch_delete=st.st_TrackBatch_delete):
         if getattr(self, "ptr", None):
-            st.st_TrackBatch_delete(self.ptr)
+            _batch_delete(self.ptr)
             self.ptr = None
         if getattr(self, "_c_buffers", None):
             self._free_buffers()

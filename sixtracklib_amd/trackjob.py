@@ -107,14 +107,16 @@ class TrackJob:
         self._c_buffers.append(p)
         return p
 
-    def _free_buffers(self):
+    def _free_buffers(self, _buffer_delete=st.st_Buffer_delete):
         for p in self._c_buffers:
-            st.st_Buffer_delete(p)
+            _buffer_delete(p)
         self._c_buffers = []
 
-    def __del__(self):
+    # the library functions are bound as defaults: at interpreter exit the
+    # module globals may already be gone when __del__ runs
+    def __del__(self, _job_delete=st.st_TrackJob_delete):
         if getattr(self, "ptr_st_track_job", None):
-            st.st_TrackJob_delete(self.ptr_st_track_job)
+            _job_delete(self.ptr_st_track_job)
             self.ptr_st_track_job = None
         if getattr(self, "_c_buffers", None):
             self._free_buffers()
