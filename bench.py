@@ -132,6 +132,47 @@ def batched_vs_sequential(n_seeds, turns):
             "autotuned_config": cfg}
 
 
+def adjoint_cost(reps=3):
+    """Differentiable form: 4,096 particles, the 1,000-element sub-lattice
+    (Drift, Multipole orders 1-3, Cavity), 10 turns, checkpoints every 64
+    elements; loss = sum over turns of |(x, px, y, py) - target(t)|^2.
+    Wall time (device synchronised) of the forward pass and of the reverse
+    pass (gradients of the 6 initial coordinates of every particle and of
+    every knl / ksl), best of ``reps``."""
+    import torch
+    from sixtracklib_amd import lattices
+    from sixtracklib_amd.adjoint import AdjointTracker
+    eb = lattices.adjoint_sublattice()
+    _, p = lattices.gaussian_beam(4096, seed=7, sigma_x=3e-4, sigma_px=3e-6, sigma_y=3e-4,
+                                  sigma_py=3e-6, sigma_zeta=0.05, sigma_delta=1e-4)
+    coords = np.stack([np.array(getattr(p, f)) for f in ("x", "px", "y", "py", "zeta",
+                                                          "delta")], 1)
+    tgt = np.random.default_rng(11).normal(0.0, [3e-5, 3e-7, 3e-5, 3e-7], (10, 4))
+    out = {}
+    for fma in (False, True):
+        tr = AdjointTracker(eb, 10, species=p, K=64, fma=fma)
+        X0 = torch.tensor(coords, dtype=torch.float64, device="cuda")
+        best_f = best_b = float("inf")
+        for _ in range(reps):
+            X = X0.clone().requires_grad_()
+            knl = tr.knl0.clone().requires_grad_()
+            ksl = tr.ksl0.clone().requires_grad_()
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            loss = tr.loss(X, knl, ksl, tgt)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            loss.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            best_f, best_b = min(best_f, t1 - t0), min(best_b, t2 - t1)
+        out["fma" if fma else "exact"] = {"forward_ms": best_f * 1e3, "backward_ms": best_b * 1e3,
+                                          "cost_ratio": best_b / best_f}
+    out.update({"particles": 4096, "elements": 1000, "turns": 10, "checkpoint_every": 64,
+                "gradients": "initial 6D coordinates + all knl/ksl (%d multipoles)" % tr.n_mp})
+    return out
+
+
 def calibration_copy(nbytes):
     """One device-to-device copy of ``nbytes`` through the HIP runtime (its
     blit kernel reads and writes exactly nbytes of HBM)."""
@@ -174,6 +215,8 @@ def main():
                     help="batched form: machine seeds (0 disables the batched measurement)")
     ap.add_argument("--batch-turns", type=int, default=100,
                     help="batched form: turns tracked by the batch and by the sequential jobs")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the differentiable-tracking cost measurement")
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device hipMemcpy of this many MiB (a copy "
                          "kernel of known traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -239,6 +282,10 @@ def main():
     if args.batch_seeds > 0:
         batched = batched_vs_sequential(args.batch_seeds, args.batch_turns)
 
+    adjoint = None
+    if not args.no_adjoint:
+        adjoint = adjoint_cost()
+
     cpu = None
     if not args.no_cpu_baseline:
         v, threads, desc = cpu_baseline(eb, pb, budget_s=args.cpu_budget)
@@ -275,6 +322,7 @@ def main():
                      "algorithmic_hbm_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "batched": batched,
+        "adjoint": adjoint,
     }
     out.update(extra)
     print(json.dumps(out))

@@ -25,7 +25,8 @@
  * still have turns to go, so lost ones take no lanes; results identical),
  * "turns_per_launch=K" (default 0: chosen from the work size; with
  * compact=1 a request is split into launches of K turns and the particle
- * list is rebuilt between them; results identical).
+ * list is rebuilt between them; results identical), "queues=1|8" (default
+ * 1: one work queue for the persistent launch; 8: one per XCD).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_JOB_H__
 #define SIXTRACKLIB_AMD_TRACK_JOB_H__

@@ -148,6 +148,9 @@ struct NS(TrackJobBase)
     int32_t* d_status = nullptr; int32_t* h_status = nullptr;
     bool compact = true;                 // lean lattices: persistent tiles over active particles
     int64_t turns_per_launch = 0;        // 0: from the work size
+    // one tile queue: a job's sets share one lattice, and with a queue per
+    // XCD the waves of the first XCD would take every tile of a small set
+    int queues = 1;
     st_batch::BatchRunner runner;        // scheduling of the lean until path
     bool runner_ready = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -391,6 +394,7 @@ struct NS(TrackJobBase)
         runner.cfg.fma = fma;
         runner.cfg.block = ( block == 64 || block == 128 ) ? block : 256;
         runner.cfg.turns_per_launch = turns_per_launch;
+        runner.cfg.queues = queues;
         runner.cfg.graph = false;
         std::vector< st_batch_seed_t > seeds;
         for( const st_dev_particles_t& v : views )
@@ -491,6 +495,7 @@ struct NS(TrackJobBase)
             else if( k == "ebe_rolling" ) ebe_rolling = iv != 0;
             else if( k == "compact" ) compact = iv != 0;
             else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
+            else if( k == "queues" ) queues = ( iv == 8 ) ? 8 : 1;
         }
     }
 };

@@ -8,6 +8,7 @@ raises ImportError -- there is no pure-Python fallback for tracking.
 from __future__ import annotations
 
 import ctypes as ct
+import importlib.util
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -17,6 +18,29 @@ if not os.path.exists(SHARED_LIB):
     raise ImportError("libsixtrack_amd.so not built: run `python -m sixtracklib_amd.build` "
                       "(or __graft_entry__.build())")
 
+
+def _share_torch_hip_runtime():
+    """PyTorch-ROCm wheels carry their own HIP runtime, which torch loads
+    by file name (libamdhip64.so next to libtorch_hip.so).  Loaded first,
+    this library would bind the system runtime (soname libamdhip64.so.7),
+    a later ``import torch`` would bring a second runtime into the process
+    and torch would then find no GPU.  Loading torch's copy first (without
+    importing torch) gives the process one runtime, shared by both.
+    SIXTRACKLIB_AMD_SYSTEM_HIP=1 skips this."""
+    if os.environ.get("SIXTRACKLIB_AMD_SYSTEM_HIP") == "1":
+        return
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return
+    if spec is None or not spec.submodule_search_locations:
+        return
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    if os.path.exists(path):
+        ct.CDLL(path, mode=ct.RTLD_GLOBAL)
+
+
+_share_torch_hip_runtime()
 sixtracklib = ct.CDLL(SHARED_LIB)
 
 st_buffer_size_t = ct.c_uint64
