@@ -17,9 +17,15 @@ roofline achieved = algorithmic FP64 flops of the tracked particle-turns
 (lattices.flops_per_particle_turn) / summed kernel time measured with HIP
 events on the job's stream; cpu_baseline = the C99 oracle with OpenMP on
 the host cores over a bounded sample of the same workload.
+
+Beside the headline figure the line carries the other forms of the path:
+"batched" (64 machine seeds in one persistent launch vs 64 plain jobs),
+"adjoint" (backward / forward cost of the differentiable tracker) and
+"elem_by_elem" (element-by-element storage through the ring with
+beam-beam and space-charge elements; an HBM-bound kernel, with its own
+roofline in GB/s).
 """
 import argparse
-import ctypes as ct
 import json
 import os
 import sys
@@ -32,6 +38,14 @@ import numpy as np  # noqa: E402
 
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 vector (FMA = 2 flops)
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (KiB per dispatch; a 1 GiB device-to-device copy reads 524,308 KiB of
+# FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
+# the 23-turn track launch of the 1M-particle ring (578,319 / 128,460 KiB)
+# and the 1-turn element-by-element launch of the collective ring with
+# 100k particles (1,789,563 / 342,027,996 KiB)
+TRACK_LAUNCH_HBM_BYTES = 578319 * 2048 + 128460 * 1024
+EBE_TURN_HBM_BYTES = 1789563 * 2048 + 342027996 * 1024
 
 
 def sigma_at_start(eb):
@@ -173,18 +187,70 @@ def adjoint_cost(reps=3):
     return out
 
 
+def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
+    """Element-by-element form: lattices.collective_ring() (20,019
+    elements: the 20k ring + 2 BeamBeam6D x 5 slices + 1 BeamBeam4D + 16
+    SCQGaussProfile), a 6D Gaussian beam matched at the ring start, every
+    element pass of every particle stored (168 B each) for ``turns`` turns
+    into a rolling store of ``passes`` passes per particle
+    (ebe_store_passes; 100k x 20,019 x 10 records would be 3.4 TB).
+    Achieved HBM rate = 168 B x records written / kernel time (HIP events
+    on the job's stream); best of ``reps`` runs from the same initial
+    state."""
+    from sixtracklib_amd import lattices
+    from sixtracklib_amd.particles import particles_in
+    from sixtracklib_amd.trackjob import TrackJob
+    eb = lattices.collective_ring()
+    pb, p = lattices.matched_gaussian_beam(n_particles)
+    init = pb.words.copy()
+    job = TrackJob(eb, pb, until_turn_elem_by_elem=turns,
+                   config_str="fma=1 ebe_store_passes=%d" % passes)
+    n_elem = eb.n_objects
+    best = None
+    for _ in range(reps):
+        pb.words[:] = init
+        job.push_particles()
+        t0 = time.perf_counter()
+        job.track_elem_by_elem(turns)
+        wall = time.perf_counter() - t0
+        kms = job.last_kernel_time_ms
+        if best is None or kms < best[1]:
+            best = (wall, kms)
+    job.collect_particles()
+    at_turn, at_elem = np.array(p.at_turn), np.array(p.at_element)
+    alive = np.array(p.state) == 1
+    # a record is stored before every element a particle enters, the one
+    # where it is lost included
+    records = int(np.sum(np.where(alive, at_turn * n_elem, at_turn * n_elem + at_elem + 1)))
+    wall, kms = best
+    del job
+    gbs = records * 168 / (kms / 1e3) / 1e9
+    return {"workload": "track_elem_by_elem, collective_ring(): %d elements (20k ring + 2 "
+                        "BeamBeam6D x 5 slices + BeamBeam4D + 16 SCQGaussProfile); %d "
+                        "particles, matched 6D Gaussian; %d turns; rolling store of %d "
+                        "passes per particle" % (n_elem, n_particles, turns, passes),
+            "particles": n_particles, "turns": turns, "elements": n_elem,
+            "store_passes": passes, "store_gb": n_particles * passes * 168 / 1e9,
+            "records": records, "alive_at_end": int(np.count_nonzero(alive)),
+            "kernel_seconds": kms / 1e3, "wall_seconds": wall,
+            "value": int(np.sum(at_turn)) / wall, "unit": "particle-turns/s",
+            "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_record": 168,
+                         "traffic_per_turn": EBE_TURN_HBM_BYTES,
+                         "algorithmic_bytes_per_turn": n_particles * n_elem * 168}}
+
+
 def calibration_copy(nbytes):
-    """One device-to-device copy of ``nbytes`` through the HIP runtime (its
-    blit kernel reads and writes exactly nbytes of HBM)."""
-    hip = ct.CDLL("libamdhip64.so")
-    a, b = ct.c_void_p(), ct.c_void_p()
-    assert hip.hipMalloc(ct.byref(a), ct.c_size_t(nbytes)) == 0
-    assert hip.hipMalloc(ct.byref(b), ct.c_size_t(nbytes)) == 0
-    assert hip.hipMemset(a, 1, ct.c_size_t(nbytes)) == 0
-    assert hip.hipMemcpy(b, a, ct.c_size_t(nbytes), 3) == 0      # hipMemcpyDeviceToDevice
-    assert hip.hipDeviceSynchronize() == 0
-    hip.hipFree(a)
-    hip.hipFree(b)
+    """One device-to-device copy of ``nbytes`` (the HIP runtime's blit
+    kernel reads and writes exactly nbytes of HBM), made through PyTorch so
+    that the process holds a single HIP runtime, the one the tracker
+    shares."""
+    import torch
+    a = torch.ones(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    del a, b
 
 
 def cpu_model():
@@ -217,6 +283,9 @@ def main():
                     help="batched form: turns tracked by the batch and by the sequential jobs")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiable-tracking cost measurement")
+    ap.add_argument("--ebe-turns", type=int, default=10,
+                    help="element-by-element form: turns stored (0 disables it)")
+    ap.add_argument("--ebe-particles", type=int, default=100000)
     ap.add_argument("--calibrate-copy-mib", type=int, default=0,
                     help="also run one device-to-device hipMemcpy of this many MiB (a copy "
                          "kernel of known traffic, to scale FETCH_SIZE / WRITE_SIZE counters)")
@@ -286,6 +355,10 @@ def main():
     if not args.no_adjoint:
         adjoint = adjoint_cost()
 
+    ebe = None
+    if args.ebe_turns > 0:
+        ebe = elem_by_elem_collective(args.ebe_particles, args.ebe_turns)
+
     cpu = None
     if not args.no_cpu_baseline:
         v, threads, desc = cpu_baseline(eb, pb, budget_s=args.cpu_budget)
@@ -318,11 +391,16 @@ def main():
                    "kernel_seconds": kernel_s, "setup_seconds": setup_s},
         "roofline": {"bound": "fp64_valu", "achieved": achieved_tflops,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved_tflops / FP64_PEAK_TFLOPS, "traffic": None,
+                     "frac": achieved_tflops / FP64_PEAK_TFLOPS,
+                     "traffic": TRACK_LAUNCH_HBM_BYTES,
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of one 23-turn launch of "
+                                       "st_k_batch_track_fma on this workload, scaled by a 1 GiB "
+                                       "device copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x)",
                      "algorithmic_hbm_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "batched": batched,
         "adjoint": adjoint,
+        "elem_by_elem": ebe,
     }
     out.update(extra)
     print(json.dumps(out))

@@ -49,6 +49,7 @@ typedef int64_t  NS(particle_index_t);
 typedef int64_t  NS(particle_num_elements_t);
 typedef int32_t  NS(track_status_t);
 typedef int32_t  NS(arch_status_t);
+typedef uint64_t NS(arch_size_t);
 typedef uint16_t NS(track_job_io_flag_t);
 typedef NS(track_job_io_flag_t) NS(track_job_collect_flag_t);
 typedef NS(track_job_io_flag_t) NS(track_job_push_flag_t);

@@ -27,6 +27,16 @@ ST_API int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,
     int64_t out_capacity, uint32_t* rec_offsets, int64_t* n_elem_out,
     uint32_t* monitor_fields, int64_t* n_monitors_out );
 
+/* As NS(Lattice_pack), with ``n_table_buffers`` further relocated CObjects
+ * images (``table_buffers``) in which the line-density table of an
+ * SCInterpolatedProfile (interpol_data_addr) may lie: the table is copied
+ * into the element's record when the profile object and its two arrays
+ * lie inside one buffer, the lattice's own or one of these. */
+ST_API int64_t NS(Lattice_pack_with_tables)( const unsigned char* belems,
+    const unsigned char* const* table_buffers, int64_t n_table_buffers, uint64_t* out,
+    int64_t out_capacity, uint32_t* rec_offsets, int64_t* n_elem_out,
+    uint32_t* monitor_fields, int64_t* n_monitors_out );
+
 #if defined( __cplusplus )
 }
 #endif

@@ -26,7 +26,14 @@
  * "turns_per_launch=K" (default 0: chosen from the work size; with
  * compact=1 a request is split into launches of K turns and the particle
  * list is rebuilt between them; results identical), "queues=1|8" (default
- * 1: one work queue for the persistent launch; 8: one per XCD).
+ * 1: one work queue for the persistent launch; 8: one per XCD),
+ * "ebe_store_passes=K" (default 0: the element-by-element store holds
+ * particles x elements x turns records, as the reference's; K > 0 sizes it
+ * for K element passes of every particle when that is fewer, and with
+ * ebe_rolling=1 it then keeps each particle's K most recent passes: record
+ * (turn t, element e, particle p) goes to slot ((t - t0) * E + e) mod K * P
+ * + p, the reference's rolling index with capacity K * P; with
+ * ebe_rolling=0 it keeps the first K).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_JOB_H__
 #define SIXTRACKLIB_AMD_TRACK_JOB_H__
@@ -41,6 +48,30 @@ extern "C" {
 
 typedef struct NS(TrackJobBase) NS(TrackJobBase);
 typedef NS(TrackJobBase) NS(TrackJob);
+
+/* Buffer ids of a job (control/definitions.h:134-168). */
+#define ST_ARCH_ILLEGAL_BUFFER_ID          ( ( NS(arch_size_t) )0u )
+#define ST_ARCH_PARTICLES_BUFFER_ID        ( ( NS(arch_size_t) )1u )
+#define ST_ARCH_BEAM_ELEMENTS_BUFFER_ID    ( ( NS(arch_size_t) )2u )
+#define ST_ARCH_OUTPUT_BUFFER_ID           ( ( NS(arch_size_t) )3u )
+#define ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ( ( NS(arch_size_t) )1024u )
+
+/* buffer/assign_address_item.h:34-45: "write the address of field
+ * src_pointer_offset of object src_elem_index (type src_elem_type_id) of
+ * buffer src_buffer_id into field dest_pointer_offset of object
+ * dest_elem_index (type dest_elem_type_id) of buffer dest_buffer_id". */
+typedef struct NS(AssignAddressItem)
+{
+    NS(buffer_size_t)    dest_buffer_id;
+    NS(buffer_size_t)    src_buffer_id;
+    NS(object_type_id_t) dest_elem_type_id;
+    NS(object_type_id_t) src_elem_type_id;
+    NS(buffer_size_t)    dest_elem_index;
+    NS(buffer_size_t)    src_elem_index;
+    NS(buffer_size_t)    dest_pointer_offset;
+    NS(buffer_size_t)    src_pointer_offset;
+}
+NS(AssignAddressItem);
 
 /* track_job.h:139 */
 ST_API NS(TrackJobBase)* NS(TrackJob_create)( char const* arch, char const* config_str );
@@ -155,12 +186,65 @@ ST_API int NS(TrackJob_uses_full_kernel)( const NS(TrackJobBase)* job );
  * own output buffer (output/output_buffer.c:44-131 + 288-363):
  * reports the particle-set sizes the job will need, in the order the
  * output buffer must hold them (element-by-element set first, then one
- * per BeamMonitor).  Returns the number of sets, -1 on error. */
+ * per BeamMonitor).  config_str (may be NULL) is the job's configuration,
+ * of which "ebe_store_passes" changes the sizes.  Returns the number of
+ * sets, -1 on error. */
 ST_API int64_t NS(TrackJob_required_output_sets)( NS(Buffer)* particles_buffer,
     NS(buffer_size_t) num_particle_sets,
     NS(buffer_size_t) const* particle_set_indices_begin,
     NS(Buffer)* beam_elements_buffer, NS(buffer_size_t) until_turn_elem_by_elem,
-    int64_t* out_sizes, int64_t max_sets, int64_t* has_elem_by_elem );
+    int64_t* out_sizes, int64_t max_sets, int64_t* has_elem_by_elem,
+    char const* config_str );
+
+/* Stored buffers (track_job.h:717-772).  A stored buffer is a relocated
+ * CObjects buffer the job keeps beside its particles, beam elements and
+ * output (ids from ST_ARCH_MIN_USER_DEFINED_BUFFER_ID on, stable when a
+ * buffer is removed).  On this device its contents reach the kernels as
+ * copies inside the packed lattice: the line-density tables of
+ * SCInterpolatedProfile elements whose interpol_data_addr points into it.
+ * push rebuilds those copies; collect has nothing to bring back.
+ * take_ownership: the job deletes the buffer (NS(Buffer_delete)) when it
+ * is removed or the job is deleted; delete_ptr_after_move is accepted and
+ * ignored (no buffer is moved). */
+ST_API NS(arch_size_t) NS(TrackJob_add_stored_buffer)( NS(TrackJobBase)* job,
+    NS(Buffer)* buffer, bool take_ownership, bool delete_ptr_after_move );
+ST_API NS(arch_size_t) NS(TrackJob_create_stored_buffer)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) buffer_capacity );
+ST_API NS(arch_status_t) NS(TrackJob_remove_stored_buffer)( NS(TrackJobBase)* job,
+    NS(arch_size_t) buffer_id );
+ST_API bool NS(TrackJob_has_stored_buffers)( const NS(TrackJobBase)* job );
+ST_API NS(arch_size_t) NS(TrackJob_num_stored_buffers)( const NS(TrackJobBase)* job );
+ST_API NS(arch_size_t) NS(TrackJob_min_stored_buffer_id)( const NS(TrackJobBase)* job );
+ST_API NS(arch_size_t) NS(TrackJob_max_stored_buffer_id)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_owns_stored_buffer)( const NS(TrackJobBase)* job, NS(arch_size_t) buffer_id );
+ST_API NS(Buffer)* NS(TrackJob_stored_buffer)( NS(TrackJobBase)* job, NS(arch_size_t) buffer_id );
+ST_API NS(arch_status_t) NS(TrackJob_push_stored_buffer)( NS(TrackJobBase)* job,
+    NS(arch_size_t) buffer_id );
+ST_API NS(arch_status_t) NS(TrackJob_collect_stored_buffer)( NS(TrackJobBase)* job,
+    NS(arch_size_t) buffer_id );
+
+/* Address assignment items (track_job.h:500-660).  Items are kept in the
+ * order added (an identical item is not added twice).  assign_* writes the
+ * host addresses into the host buffers and, when beam elements changed,
+ * rebuilds the device lattice.  Destinations other than the beam-element
+ * buffer are rejected (status -1, message in NS(Hip_last_error_str)): the
+ * job prepares its own output addresses. */
+ST_API NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item)( NS(TrackJobBase)* job,
+    const NS(AssignAddressItem)* assign_item_to_add );
+ST_API NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item_detailed)(
+    NS(TrackJobBase)* job, NS(object_type_id_t) dest_type_id, NS(buffer_size_t) dest_buffer_id,
+    NS(buffer_size_t) dest_elem_index, NS(buffer_size_t) dest_pointer_offset,
+    NS(object_type_id_t) src_type_id, NS(buffer_size_t) src_buffer_id,
+    NS(buffer_size_t) src_elem_index, NS(buffer_size_t) src_pointer_offset );
+ST_API bool NS(TrackJob_has_assign_items)( const NS(TrackJobBase)* job,
+    NS(arch_size_t) dest_buffer_id, NS(arch_size_t) src_buffer_id );
+ST_API NS(arch_size_t) NS(TrackJob_num_assign_items)( const NS(TrackJobBase)* job,
+    NS(arch_size_t) dest_buffer_id, NS(arch_size_t) src_buffer_id );
+ST_API NS(arch_size_t) NS(TrackJob_total_num_assign_items)( const NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_commit_address_assignments)( NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_assign_all_addresses)( NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_assign_addresses)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) dest_buffer_id, NS(buffer_size_t) src_buffer_id );
 
 /* Device/runtime introspection (no reference counterpart). */
 ST_API int NS(Hip_num_devices)( void );

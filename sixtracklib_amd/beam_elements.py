@@ -495,9 +495,17 @@ class SCInterpolatedProfile(CObject):
 
 
 class LineDensityProfileData(CObject):
-    """be_beamfields/be_beamfields.h:887-896 (method 0 linear, 1 cubic,
-    255 none; values and derivatives arrays of ``capacity``)."""
+    """be_beamfields/be_beamfields.h:887-896; python/sixtracklib/
+    sc_line_density.py:31-112.  Line density on the grid z0 + dz * i,
+    i < num_values: ``values`` and ``derivatives`` arrays of ``capacity``;
+    ``method`` 0 linear (derivatives = forward differences), 1 cubic
+    (derivatives = second derivatives of the natural cubic spline), 255
+    none (the element falls back to its constant density)."""
     _typeid = 258
+    INTERPOL_LINEAR = 0
+    INTERPOL_CUBIC = 1
+    INTERPOL_NONE = 255
+
     method = CField(0, "int64", default=0)
     capacity = CField(1, "int64", default=2, const=True)
     values = CField(2, "real", default=0.0, pointer=True, length="capacity")
@@ -506,6 +514,95 @@ class LineDensityProfileData(CObject):
     z0 = CField(4, "real", default=0.0)
     dz = CField(5, "real", default=1.0)
     num_values = CField(6, "int64", default=0)
+
+    def __init__(self, method=None, capacity=None, num_values=None, **kwargs):
+        if isinstance(method, str):
+            method = {"linear": self.INTERPOL_LINEAR, "cubic": self.INTERPOL_CUBIC}.get(
+                method.lower().strip(), self.INTERPOL_NONE)
+        if method is not None:
+            kwargs["method"] = method
+        if capacity is not None or num_values is not None:
+            capacity = 1 if capacity is None else capacity
+            num_values = capacity if num_values is None else num_values
+            kwargs["capacity"] = max(capacity, num_values)
+            kwargs["num_values"] = num_values
+        super().__init__(**kwargs)
+
+    @staticmethod
+    def ptr_offset():
+        """Byte offset an address assignment points at (the object itself)."""
+        return 0
+
+    def prepare_interpolation(self):
+        """Fill ``derivatives`` for the object's method
+        (internal/math_interpol.h:322-600, default boundaries): forward
+        differences for linear; for cubic the second derivatives of the
+        natural spline, from the tridiagonal system solved by forward
+        elimination and back substitution in that file's order."""
+        n = int(self.num_values)
+        y = np.array(self.values[:n], dtype=np.float64)
+        dx = float(self.dz)
+        if not (n > 1 and dx > 0.0 and math.isfinite(dx) and math.isfinite(float(self.z0))):
+            raise RuntimeError("unable to calculate derivatives for interpolation")
+        out = np.array(self.derivatives, dtype=np.float64)
+        if int(self.method) == self.INTERPOL_LINEAR:
+            out[: n - 1] = (y[1:] - y[:-1]) / dx
+        elif int(self.method) == self.INTERPOL_CUBIC:
+            if n <= 3:
+                raise RuntimeError("cubic interpolation needs more than 3 values")
+            out[:n] = _natural_spline_ypp(y, dx)
+        else:
+            raise RuntimeError("unable to calculate derivatives for interpolation")
+        self.derivatives = out
+
+    def interpol(self, z):
+        """Interpolated line density at ``z`` (be_beamfields.h:3385-3410):
+        0 outside [z0, z0 + dz (num_values - 1))."""
+        n, z0, dz = int(self.num_values), float(self.z0), float(self.dz)
+        if n <= 0 or int(self.method) not in (self.INTERPOL_LINEAR, self.INTERPOL_CUBIC):
+            return 0.0
+        if not (z0 <= z <= z0 + dz * n):
+            return 0.0
+        i = int((z - z0) / dz)
+        if i + 1 >= n:
+            return 0.0
+        y, yd = self.values, self.derivatives
+        if int(self.method) == self.INTERPOL_LINEAR:
+            return float(y[i] + (z - (z0 + dz * i)) * yd[i])
+        t2 = (z0 + dz * (i + 1) - z) / dz
+        t1 = (z - (z0 + dz * i)) / dz
+        r = y[i + 1] * t1 + y[i] * t2
+        r -= yd[i] * dz * dz * (t2 - t2 * t2 * t2) / 6.0
+        r -= yd[i + 1] * dz * dz * (t1 - t1 * t1 * t1) / 6.0
+        return float(r)
+
+
+def _natural_spline_ypp(y, dx):
+    """Second derivatives at the nodes of the natural cubic spline through
+    y on a grid of step dx (internal/math_interpol.h:369-600: h = dx,
+    g = 4 dx, right-hand side 6 (forward - backward difference quotient),
+    Thomas elimination, then zero second derivative at both ends)."""
+    n = len(y)
+    g = [0.0] * n
+    dy = [0.0] * n
+    for i in range(1, n - 1):
+        g[i] = (dx + dx) * 2.0
+        dy[i] = ((y[i + 1] - y[i]) / dx - (y[i] - y[i - 1]) / dx) * 6.0
+    d = [0.0] * n
+    ypp = [0.0] * n
+    d[0] = g[1]
+    ypp[0] = dy[1]
+    for i in range(1, n - 2):
+        l_i = dx / d[i - 1]
+        d[i] = g[i + 1] - l_i * dx
+        ypp[i] = dy[i + 1] - ypp[i - 1] * l_i
+    i = n - 3
+    ypp[i + 1] = ypp[i] / d[i]
+    for i in range(n - 4, -1, -1):
+        ypp[i + 1] = (ypp[i] - dx * ypp[i + 2]) / d[i]
+    ypp[0] = 0.0
+    ypp[n - 1] = 0.0
+    return np.array(ypp)
 
 
 class Elements:

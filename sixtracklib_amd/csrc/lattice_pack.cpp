@@ -7,6 +7,7 @@
 // which are disabled, which mark the particle lost).
 #include <cstdint>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "sixtracklib_amd/definitions.h"
@@ -38,9 +39,23 @@ struct Packer
     std::vector< uint32_t > offs;
     std::vector< uint32_t > monitor_fields;   // word index of out_dev per monitor
     uint64_t lo = 0, hi = 0;                  // buffer extent (bounds for pointers)
+    // extents of further buffers that may hold line-density tables
+    std::vector< std::pair< uint64_t, uint64_t > > tables;
 
     bool inside( uint64_t addr, uint64_t bytes ) const
     { return addr >= lo && addr + bytes <= hi && ( addr % 8u ) == 0u; }
+
+    static bool within( const std::pair< uint64_t, uint64_t >& r, uint64_t addr, uint64_t bytes )
+    { return addr >= r.first && addr + bytes <= r.second && ( addr % 8u ) == 0u; }
+
+    // extent (this buffer or a table buffer) holding [addr, addr + bytes);
+    // {0, 0} when none does
+    std::pair< uint64_t, uint64_t > extent_of( uint64_t addr, uint64_t bytes ) const
+    {
+        if( inside( addr, bytes ) ) return { lo, hi };
+        for( const auto& r : tables ) if( within( r, addr, bytes ) ) return r;
+        return { 0u, 0u };
+    }
 
     uint64_t index = 0;                       // position of the object being packed
 
@@ -150,15 +165,18 @@ struct Packer
         case ST_OBJECT_TYPE_SC_INTERPOLATED_PROF:
         {
             std::vector< uint64_t > pl( e, e + 10 );
-            // line-density table inlined when the profile object lies in
-            // this buffer (be_beamfields.h:887-896); otherwise method 255
-            // ("none") selects the fallback density like a missing profile
+            // line-density table (be_beamfields.h:887-896) inlined when the
+            // profile object and its arrays lie in this buffer or in one of
+            // the table buffers; otherwise method 255 ("none") selects the
+            // fallback density like a missing profile
             uint64_t const ld = e[ 6 ];
-            bool ok = ld != 0u && inside( ld, 56u );
+            auto const ext = ld != 0u ? extent_of( ld, 56u ) : std::pair< uint64_t, uint64_t >{ 0u, 0u };
+            bool ok = ext.second != 0u;
             const uint64_t* L = ok ? at( ld ) : nullptr;
             int64_t nv = 0;
             if( ok ) { std::memcpy( &nv, &L[ 6 ], 8 );
-                       ok = nv > 0 && nv < ( 1 << 24 ) && inside( L[ 2 ], 8u * nv ) && inside( L[ 3 ], 8u * nv ); }
+                       ok = nv > 0 && nv < ( 1 << 24 ) && within( ext, L[ 2 ], 8u * nv ) &&
+                            within( ext, L[ 3 ], 8u * nv ); }
             if( ok )
             {
                 pl.push_back( L[ 0 ] ); pl.push_back( static_cast< uint64_t >( nv ) );
@@ -184,11 +202,13 @@ struct Packer
 };
 }  // namespace
 
-extern "C" int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,
+extern "C" int64_t NS(Lattice_pack_with_tables)( const unsigned char* belems,
+    const unsigned char* const* table_buffers, int64_t n_table_buffers, uint64_t* out,
     int64_t out_capacity, uint32_t* rec_offsets, int64_t* n_elem_out,
     uint32_t* monitor_fields, int64_t* n_monitors_out )
 {
-    if( belems == nullptr ) return -1;
+    if( belems == nullptr || n_table_buffers < 0 ||
+        ( n_table_buffers > 0 && table_buffers == nullptr ) ) return -1;
     const uint64_t* h = reinterpret_cast< const uint64_t* >( belems );
     if( h[ 0 ] != reinterpret_cast< uint64_t >( belems ) ) return -1;   // not relocated
     int64_t n = 0;
@@ -196,6 +216,12 @@ extern "C" int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,
     Packer pk;
     pk.lo = h[ 0 ];
     pk.hi = h[ 0 ] + h[ 1 ];
+    for( int64_t k = 0; k < n_table_buffers; ++k )
+    {
+        const uint64_t* t = reinterpret_cast< const uint64_t* >( table_buffers[ k ] );
+        if( t == nullptr || t[ 0 ] != reinterpret_cast< uint64_t >( t ) ) return -1;
+        pk.tables.emplace_back( t[ 0 ], t[ 0 ] + t[ 1 ] );
+    }
     pk.offs.reserve( n + 1 );
     for( int64_t i = 0; i < n; ++i )
     {
@@ -219,4 +245,12 @@ extern "C" int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,
             std::memcpy( monitor_fields, pk.monitor_fields.data(), sizeof( uint32_t ) * pk.monitor_fields.size() );
     }
     return nw;
+}
+
+extern "C" int64_t NS(Lattice_pack)( const unsigned char* belems, uint64_t* out,
+    int64_t out_capacity, uint32_t* rec_offsets, int64_t* n_elem_out,
+    uint32_t* monitor_fields, int64_t* n_monitors_out )
+{
+    return NS(Lattice_pack_with_tables)( belems, nullptr, 0, out, out_capacity, rec_offsets,
+                                         n_elem_out, monitor_fields, n_monitors_out );
 }

@@ -89,23 +89,48 @@ bool find_attributes( NS(Buffer)* pbuf, const std::vector< uint64_t >& sets,
     return true;
 }
 
+// integer value of ``key=`` in a configuration string (tokens separated by
+// blanks, commas or semicolons), ``def`` when absent
+long config_int( const char* cfg, const char* key, long def )
+{
+    if( cfg == nullptr ) return def;
+    std::string s = cfg;
+    for( char& ch : s ) if( ch == ',' || ch == ';' ) ch = ' ';
+    std::string const k = std::string( key ) + "=";
+    size_t pos = 0;
+    long v = def;
+    while( ( pos = s.find( k, pos ) ) != std::string::npos )
+    {
+        if( pos == 0 || s[ pos - 1 ] == ' ' )
+            v = std::strtol( s.c_str() + pos + k.size(), nullptr, 10 );
+        pos += k.size();
+    }
+    return v;
+}
+
 struct OutputPlan
 {
     bool has_ebe = false;
     int64_t ebe_size = 0, max_ebe_turn = 0;
+    int64_t ebe_cap = 0;                       // store capacity in records
     std::vector< int64_t > monitor_elems;      // element index of every monitor
     std::vector< int64_t > monitor_sizes;
 };
 
-// output/output_buffer.c:288-363
-void plan_output( NS(Buffer)* ebuf, const Attr& a, uint64_t until_turn_ebe, OutputPlan& o )
+// output/output_buffer.c:288-363.  ``ebe_passes`` > 0 sizes the
+// element-by-element store for that many element passes of every particle
+// when that is fewer than the reference's (elements x turns).
+void plan_output( NS(Buffer)* ebuf, const Attr& a, uint64_t until_turn_ebe, int64_t ebe_passes,
+                  OutputPlan& o )
 {
     int64_t const np = a.max_pid - a.min_pid + 1;
     if( a.min_turn >= 0 && ( int64_t )until_turn_ebe > a.min_turn )
     {
         o.has_ebe = true;
         o.max_ebe_turn = ( int64_t )until_turn_ebe - 1;
-        o.ebe_size = np * ( a.max_elem - a.min_elem + 1 ) * ( o.max_ebe_turn - a.min_turn + 1 );
+        int64_t const passes = ( a.max_elem - a.min_elem + 1 ) * ( o.max_ebe_turn - a.min_turn + 1 );
+        o.ebe_size = np * ( ( ebe_passes > 0 && ebe_passes < passes ) ? ebe_passes : passes );
+        o.ebe_cap = o.ebe_size;
     }
     int64_t const n = ( int64_t )NS(Buffer_get_num_of_objects)( ebuf );
     for( int64_t i = 0; i < n; ++i )
@@ -134,6 +159,7 @@ struct NS(TrackJobBase)
     uint64_t ebe_offset = 0, monitor_offset = 0, until_turn_ebe = 0;
     st_ebe_config_t ebe_cfg{};
     bool ebe_rolling = true, fma = false;
+    int64_t ebe_passes = 0;              // 0: the full (elements x turns) store
     int block = 256, device = 0;
     uint16_t collect_flags = ST_TRACK_JOB_COLLECT_DEFAULT_FLAGS;
     int32_t last_status = 0;
@@ -158,7 +184,62 @@ struct NS(TrackJobBase)
     std::vector< st_dev_particles_t > views;
     st_dev_particles_t ebe_view{};
 
-    ~NS(TrackJobBase)() { release(); }
+    ~NS(TrackJobBase)()
+    {
+        release();
+        for( Stored& b : stored ) if( b.owned && b.buf ) NS(Buffer_delete)( b.buf );
+    }
+
+    // -- stored buffers and address assignments (track_job_base.cpp:
+    // 1133-1260, 2748-2805): buffer ids 1 particles, 2 beam elements,
+    // 3 output, 1024 + k the k-th stored buffer
+    struct Stored { NS(Buffer)* buf = nullptr; bool owned = false; };
+    std::vector< Stored > stored;
+    std::vector< NS(AssignAddressItem) > assign_items;
+
+    NS(Buffer)* buffer_by_id( uint64_t id ) const
+    {
+        if( id == ST_ARCH_PARTICLES_BUFFER_ID ) return pbuf;
+        if( id == ST_ARCH_BEAM_ELEMENTS_BUFFER_ID ) return ebuf;
+        if( id == ST_ARCH_OUTPUT_BUFFER_ID ) return obuf;
+        if( id >= ST_ARCH_MIN_USER_DEFINED_BUFFER_ID &&
+            id - ST_ARCH_MIN_USER_DEFINED_BUFFER_ID < stored.size() )
+            return stored[ id - ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ].buf;
+        return nullptr;
+    }
+
+    // address of field ``offset`` of object ``index`` of type ``type`` in
+    // ``b`` (host address), 0 when there is no such object / field
+    static uint64_t field_addr( NS(Buffer)* b, uint64_t type, uint64_t index, uint64_t offset,
+                                uint64_t width )
+    {
+        if( b == nullptr || index >= NS(Buffer_get_num_of_objects)( b ) ) return 0u;
+        NS(Object) const* o = NS(Buffer_get_const_object)( b, index );
+        if( o == nullptr || o->type_id != type || offset + width > o->size ) return 0u;
+        return o->begin_addr + offset;
+    }
+
+    // performs the items with the given (dest, src) pair, or all of them;
+    // the device lattice is rebuilt when a beam element was changed
+    bool assign( bool all, uint64_t dest_id, uint64_t src_id )
+    {
+        bool lattice_changed = false;
+        for( const NS(AssignAddressItem)& it : assign_items )
+        {
+            if( !all && ( it.dest_buffer_id != dest_id || it.src_buffer_id != src_id ) ) continue;
+            if( it.dest_buffer_id != ST_ARCH_BEAM_ELEMENTS_BUFFER_ID )
+            { g_err = "address assignments into the particle or output buffer are not supported"; return false; }
+            uint64_t const dst = field_addr( buffer_by_id( it.dest_buffer_id ), it.dest_elem_type_id,
+                                             it.dest_elem_index, it.dest_pointer_offset, 8u );
+            uint64_t const src = field_addr( buffer_by_id( it.src_buffer_id ), it.src_elem_type_id,
+                                             it.src_elem_index, it.src_pointer_offset, 0u );
+            if( dst == 0u || src == 0u || ( dst % 8u ) != 0u )
+            { g_err = "address assignment item does not name existing objects"; return false; }
+            std::memcpy( reinterpret_cast< void* >( static_cast< uintptr_t >( dst ) ), &src, 8 );
+            lattice_changed = true;
+        }
+        return !lattice_changed || d_lat == nullptr || pack_lattice();
+    }
 
     void release()
     {
@@ -222,15 +303,22 @@ struct NS(TrackJobBase)
         runner_ready = false;                   // seeds point at the lattice
         int64_t n_elem_l = 0, n_mon = 0;
         const unsigned char* eb = NS(Buffer_get_const_data_begin)( ebuf );
-        int64_t const nw = NS(Lattice_pack)( eb, nullptr, 0, nullptr, &n_elem_l, nullptr, &n_mon );
+        // stored buffers may hold line-density tables of space-charge elements
+        std::vector< const unsigned char* > tables;
+        for( const Stored& b : stored )
+            if( b.buf ) tables.push_back( NS(Buffer_get_const_data_begin)( b.buf ) );
+        int64_t const nt = ( int64_t )tables.size();
+        int64_t const nw = NS(Lattice_pack_with_tables)( eb, tables.data(), nt, nullptr, 0, nullptr,
+                                                         &n_elem_l, nullptr, &n_mon );
         if( nw < 0 ) { g_err = "lattice could not be packed"; return false; }
         // zero padding: the kernels read the header one record ahead and
         // fetch multipole coefficients in batches of up to 16 words
         lat_host.assign( nw + 32, 0u );
         rec_offsets.assign( n_elem_l + 1, 0u );
         monitor_fields.assign( n_mon, 0u );
-        if( NS(Lattice_pack)( eb, lat_host.data(), nw, rec_offsets.data(), &n_elem_l,
-                              monitor_fields.data(), &n_mon ) != nw ) return false;
+        if( NS(Lattice_pack_with_tables)( eb, tables.data(), nt, lat_host.data(), nw,
+                                          rec_offsets.data(), &n_elem_l, monitor_fields.data(),
+                                          &n_mon ) != nw ) return false;
         n_elem = n_elem_l;
         lat_words = nw;
         lat_full = false;
@@ -332,7 +420,7 @@ struct NS(TrackJobBase)
         attr = Attr();
         plan = OutputPlan();
         if( !find_attributes( pbuf, sets, ebuf, attr, nullptr ) ) return false;
-        plan_output( ebuf, attr, until_turn_ebe, plan );
+        plan_output( ebuf, attr, until_turn_ebe, ebe_passes, plan );
         if( !prepare_output() ) return false;
 
         if( !hip_ok( hipSetDevice( device ), "hipSetDevice" ) ) return false;
@@ -441,8 +529,8 @@ struct NS(TrackJobBase)
         for( const st_dev_particles_t& v : views )
         {
             hipError_t const err = fma
-                ? st_launch_track_ebe_fma( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), &ebe_view, d_status, block, stream )
-                : st_launch_track_ebe_exact( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), &ebe_view, d_status, block, stream );
+                ? st_launch_track_ebe_fma( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), plan.ebe_cap, &ebe_view, d_status, block, stream )
+                : st_launch_track_ebe_exact( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), plan.ebe_cap, &ebe_view, d_status, block, stream );
             ok = ok && hip_ok( err, "launch track_elem_by_elem" );
         }
         return end_status( ok );
@@ -493,6 +581,7 @@ struct NS(TrackJobBase)
             else if( k == "block" ) block = ( int )iv;
             else if( k == "device" ) { device = ( int )iv; device_id = v; }
             else if( k == "ebe_rolling" ) ebe_rolling = iv != 0;
+            else if( k == "ebe_store_passes" ) ebe_passes = iv > 0 ? iv : 0;
             else if( k == "compact" ) compact = iv != 0;
             else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
             else if( k == "queues" ) queues = ( iv == 8 ) ? 8 : 1;
@@ -697,7 +786,7 @@ int NS(TrackJob_uses_full_kernel)( const NS(TrackJobBase)* j ) { return ( j && j
 
 int64_t NS(TrackJob_required_output_sets)( NS(Buffer)* pbuf, NS(buffer_size_t) nsets,
     NS(buffer_size_t) const* idx, NS(Buffer)* ebuf, NS(buffer_size_t) ute,
-    int64_t* out_sizes, int64_t max_sets, int64_t* has_ebe )
+    int64_t* out_sizes, int64_t max_sets, int64_t* has_ebe, char const* config_str )
 {
     if( pbuf == nullptr || ebuf == nullptr ) return -1;
     std::vector< uint64_t > sets;
@@ -705,7 +794,8 @@ int64_t NS(TrackJob_required_output_sets)( NS(Buffer)* pbuf, NS(buffer_size_t) n
     Attr a;
     OutputPlan plan;
     if( !find_attributes( pbuf, sets, ebuf, a, nullptr ) ) return -1;
-    plan_output( ebuf, a, ute, plan );
+    long const passes = config_int( config_str, "ebe_store_passes", 0 );
+    plan_output( ebuf, a, ute, passes > 0 ? passes : 0, plan );
     std::vector< int64_t > need;
     if( plan.has_ebe ) need.push_back( plan.ebe_size );
     need.insert( need.end(), plan.monitor_sizes.begin(), plan.monitor_sizes.end() );
@@ -726,5 +816,146 @@ char const* NS(Hip_last_error_str)( void ) { return g_err.c_str(); }
 
 // library-internal: other translation units (batch.cpp) report errors here
 void st_internal_set_error( char const* msg ) { g_err = msg ? msg : ""; }
+
+// -- stored buffers (track_job.h:717-772) --------------------------------
+NS(arch_size_t) NS(TrackJob_add_stored_buffer)( NS(TrackJobBase)* j, NS(Buffer)* buffer,
+    bool take_ownership, bool delete_ptr_after_move )
+{
+    ( void )delete_ptr_after_move;
+    if( j == nullptr || buffer == nullptr ) return ST_ARCH_ILLEGAL_BUFFER_ID;
+    if( NS(Buffer_needs_remapping)( buffer ) )
+    { g_err = "stored buffers must not need remapping"; return ST_ARCH_ILLEGAL_BUFFER_ID; }
+    j->stored.push_back( { buffer, take_ownership } );
+    return ST_ARCH_MIN_USER_DEFINED_BUFFER_ID + j->stored.size() - 1u;
+}
+
+NS(arch_size_t) NS(TrackJob_create_stored_buffer)( NS(TrackJobBase)* j,
+    NS(buffer_size_t) buffer_capacity )
+{
+    if( j == nullptr ) return ST_ARCH_ILLEGAL_BUFFER_ID;
+    NS(Buffer)* b = NS(Buffer_new)( buffer_capacity );
+    if( b == nullptr ) return ST_ARCH_ILLEGAL_BUFFER_ID;
+    j->stored.push_back( { b, true } );
+    return ST_ARCH_MIN_USER_DEFINED_BUFFER_ID + j->stored.size() - 1u;
+}
+
+NS(arch_status_t) NS(TrackJob_remove_stored_buffer)( NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    if( j == nullptr || id < ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ||
+        id - ST_ARCH_MIN_USER_DEFINED_BUFFER_ID >= j->stored.size() ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    auto& b = j->stored[ id - ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ];
+    if( b.buf == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    if( b.owned ) NS(Buffer_delete)( b.buf );
+    b.buf = nullptr; b.owned = false;               // ids of the others stay valid
+    return ST_ARCH_STATUS_SUCCESS;
+}
+
+NS(arch_size_t) NS(TrackJob_num_stored_buffers)( const NS(TrackJobBase)* j )
+{
+    NS(arch_size_t) n = 0;
+    if( j ) for( const auto& b : j->stored ) n += b.buf != nullptr ? 1u : 0u;
+    return n;
+}
+
+bool NS(TrackJob_has_stored_buffers)( const NS(TrackJobBase)* j ) { return NS(TrackJob_num_stored_buffers)( j ) > 0u; }
+
+NS(arch_size_t) NS(TrackJob_min_stored_buffer_id)( const NS(TrackJobBase)* j )
+{
+    return NS(TrackJob_has_stored_buffers)( j ) ? ST_ARCH_MIN_USER_DEFINED_BUFFER_ID : ST_ARCH_ILLEGAL_BUFFER_ID;
+}
+
+NS(arch_size_t) NS(TrackJob_max_stored_buffer_id)( const NS(TrackJobBase)* j )
+{
+    return NS(TrackJob_has_stored_buffers)( j )
+        ? ST_ARCH_MIN_USER_DEFINED_BUFFER_ID + j->stored.size() - 1u : ST_ARCH_ILLEGAL_BUFFER_ID;
+}
+
+bool NS(TrackJob_owns_stored_buffer)( const NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    if( j == nullptr || id < ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ||
+        id - ST_ARCH_MIN_USER_DEFINED_BUFFER_ID >= j->stored.size() ) return false;
+    auto const& b = j->stored[ id - ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ];
+    return b.buf != nullptr && b.owned;
+}
+
+NS(Buffer)* NS(TrackJob_stored_buffer)( NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    return ( j && id >= ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ) ? j->buffer_by_id( id ) : nullptr;
+}
+
+// the device holds stored data only as copies inside the packed lattice:
+// a push rebuilds it, a collect has nothing to bring back
+NS(arch_status_t) NS(TrackJob_push_stored_buffer)( NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    if( NS(TrackJob_stored_buffer)( j, id ) == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    return ( j->d_lat == nullptr || j->pack_lattice() ) ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
+}
+
+NS(arch_status_t) NS(TrackJob_collect_stored_buffer)( NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    return NS(TrackJob_stored_buffer)( j, id ) != nullptr ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
+}
+
+// -- address assignment items (track_job.h:500-660) -----------------------
+NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item)( NS(TrackJobBase)* j,
+    const NS(AssignAddressItem)* item )
+{
+    if( j == nullptr || item == nullptr ) return nullptr;
+    for( NS(AssignAddressItem)& it : j->assign_items )
+        if( std::memcmp( &it, item, sizeof( it ) ) == 0 ) return &it;   // already there
+    j->assign_items.push_back( *item );
+    return &j->assign_items.back();
+}
+
+NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item_detailed)( NS(TrackJobBase)* j,
+    NS(object_type_id_t) dest_type_id, NS(buffer_size_t) dest_buffer_id,
+    NS(buffer_size_t) dest_elem_index, NS(buffer_size_t) dest_pointer_offset,
+    NS(object_type_id_t) src_type_id, NS(buffer_size_t) src_buffer_id,
+    NS(buffer_size_t) src_elem_index, NS(buffer_size_t) src_pointer_offset )
+{
+    NS(AssignAddressItem) it{};
+    it.dest_buffer_id = dest_buffer_id; it.src_buffer_id = src_buffer_id;
+    it.dest_elem_type_id = dest_type_id; it.src_elem_type_id = src_type_id;
+    it.dest_elem_index = dest_elem_index; it.src_elem_index = src_elem_index;
+    it.dest_pointer_offset = dest_pointer_offset; it.src_pointer_offset = src_pointer_offset;
+    return NS(TrackJob_add_assign_address_item)( j, &it );
+}
+
+NS(arch_size_t) NS(TrackJob_num_assign_items)( const NS(TrackJobBase)* j,
+    NS(arch_size_t) dest_buffer_id, NS(arch_size_t) src_buffer_id )
+{
+    NS(arch_size_t) n = 0;
+    if( j ) for( const auto& it : j->assign_items )
+        n += ( it.dest_buffer_id == dest_buffer_id && it.src_buffer_id == src_buffer_id ) ? 1u : 0u;
+    return n;
+}
+
+bool NS(TrackJob_has_assign_items)( const NS(TrackJobBase)* j, NS(arch_size_t) dest_buffer_id,
+    NS(arch_size_t) src_buffer_id )
+{
+    return NS(TrackJob_num_assign_items)( j, dest_buffer_id, src_buffer_id ) > 0u;
+}
+
+NS(arch_size_t) NS(TrackJob_total_num_assign_items)( const NS(TrackJobBase)* j )
+{
+    return j ? j->assign_items.size() : 0u;
+}
+
+NS(arch_status_t) NS(TrackJob_commit_address_assignments)( NS(TrackJobBase)* j )
+{
+    return j ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
+}
+
+NS(arch_status_t) NS(TrackJob_assign_all_addresses)( NS(TrackJobBase)* j )
+{
+    return ( j && j->assign( true, 0u, 0u ) ) ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
+}
+
+NS(arch_status_t) NS(TrackJob_assign_addresses)( NS(TrackJobBase)* j,
+    NS(buffer_size_t) dest_buffer_id, NS(buffer_size_t) src_buffer_id )
+{
+    return ( j && j->assign( false, dest_buffer_id, src_buffer_id ) )
+        ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
+}
 
 }  // extern "C"

@@ -81,6 +81,26 @@ ST_DEV void store_record( const st_dev_particles_t& O, int64_t k,
     O.state[ k ] = L.state;
 }
 
+// store_record for record k < 2^29 of every field array: one 32-bit byte
+// offset serves all 21 stores (scalar base + vector offset addressing)
+ST_DEV void store_record_narrow( const st_dev_particles_t& O, int64_t k, double gamma0,
+    const Lane& L, const LaneConst& C, int64_t pid, int64_t at_element, int64_t at_turn )
+{
+    uint32_t const b = static_cast< uint32_t >( k ) * 8u;
+    auto d = [ b ]( double* base ) -> double&
+    { return *reinterpret_cast< double* >( reinterpret_cast< char* >( base ) + b ); };
+    auto n = [ b ]( int64_t* base ) -> int64_t&
+    { return *reinterpret_cast< int64_t* >( reinterpret_cast< char* >( base ) + b ); };
+    d( O.q0 ) = C.q0; d( O.mass0 ) = C.mass0; d( O.beta0 ) = C.beta0;
+    d( O.gamma0 ) = gamma0; d( O.p0c ) = C.p0c; d( O.s ) = L.s;
+    d( O.x ) = L.x; d( O.y ) = L.y; d( O.px ) = L.px; d( O.py ) = L.py;
+    d( O.zeta ) = L.zeta; d( O.psigma ) = L.psigma; d( O.delta ) = L.delta;
+    d( O.rpp ) = L.rpp; d( O.rvv ) = L.rvv; d( O.chi ) = C.chi;
+    d( O.charge_ratio ) = C.charge_ratio; n( O.particle_id ) = pid;
+    n( O.at_element ) = at_element; n( O.at_turn ) = at_turn;
+    n( O.state ) = L.state;
+}
+
 // device copy of an output particle set header, addressed by a monitor
 ST_DEV st_dev_particles_t load_set( uint64_t addr )
 {
@@ -167,9 +187,12 @@ ST_DEV void step_other( unsigned op, unsigned aux, cptr_t e, Lane& L, const Lane
     fail = true;                                          // track.h:577-581
 }
 
-// output/elem_by_elem_config.h:663-760
-ST_DEV int64_t ebe_index( const st_ebe_config_t& c, int64_t pid, int64_t at_element,
-                          int64_t at_turn )
+// output/elem_by_elem_config.h:663-760.  ``cap`` is the store's capacity
+// in records: num_particles * num_elements * num_turns as in the reference,
+// or a smaller multiple of num_particles for a store that keeps only the
+// most recent element passes of every particle (rolling).
+ST_DEV int64_t ebe_index( const st_ebe_config_t& c, int64_t cap, int64_t pid,
+                          int64_t at_element, int64_t at_turn )
 {
     int64_t out = -1;
     if( c.min_particle_id <= pid && c.max_particle_id >= pid &&
@@ -177,7 +200,6 @@ ST_DEV int64_t ebe_index( const st_ebe_config_t& c, int64_t pid, int64_t at_elem
         c.min_turn <= at_turn && ( c.max_turn >= at_turn || c.is_rolling ) )
     {
         int64_t const np = c.num_particles_to_store, ne = c.num_elements_to_store;
-        int64_t const cap = np * ne * c.num_turns_to_store;
         if( c.order == 0 )
         {
             out = ( at_turn - c.min_turn ) * np * ne +
@@ -195,6 +217,7 @@ struct Args
     int64_t until_turn;
     bool finish_turn;        // line mode
     st_ebe_config_t cfg;     // ebe mode
+    int64_t ebe_cap;         // ebe store capacity in records
     st_dev_particles_t O;    // ebe output set
     int32_t* status_out;
 };
@@ -227,6 +250,15 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
         if( n_elem <= 0 ) status = -1;            // empty lattice: failure
     }
 
+    // element by element: the store slot advances by num_particles per
+    // element pass (wrapping at the capacity when rolling), so the full
+    // index is computed once per turn, at the first pass inside the
+    // configured element range; records below 2^29 use 32-bit offsets
+    double const gamma0 = MODE == MODE_EBE ? P.gamma0[ li ] : 0.0;
+    bool const narrow = MODE == MODE_EBE && a.O.n <= ( int64_t( 1 ) << 29 );
+    bool const pid_ok = MODE == MODE_EBE && a.cfg.min_particle_id <= pid &&
+                        a.cfg.max_particle_id >= pid;
+
     // lanes still being tracked; a lane leaves the mask exactly once, after
     // its final state has been stored
     uint64_t act = ballot( active0 );
@@ -235,6 +267,9 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
         cptr_t r = a.lat;
         uint64_t h = r[ 0 ];
         int e_end = n_elem;
+        bool const turn_ok = pid_ok && a.cfg.min_turn <= at_turn &&
+                             ( a.cfg.max_turn >= at_turn || a.cfg.is_rolling );
+        int64_t slot = -1;                        // store slot of the next pass
         for( int e = 0; e < e_end; ++e )
         {
             cptr_t const nr = r + ST_REC_LEN( h );
@@ -243,9 +278,23 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
             cptr_t const prm = r + ST_REC_PAYLOAD;
             if( MODE == MODE_EBE )
             {
-                int64_t const k = ebe_index( a.cfg, pid, start + e, at_turn );
+                int64_t const el = start + e;
+                int64_t k = -1;
+                if( turn_ok && a.cfg.min_element_id <= el && a.cfg.max_element_id >= el )
+                {
+                    if( slot < 0 ) slot = ebe_index( a.cfg, a.ebe_cap, pid, el, at_turn );
+                    k = slot;
+                    if( slot >= 0 )
+                    {
+                        slot += a.cfg.num_particles_to_store;
+                        if( a.cfg.is_rolling && slot >= a.ebe_cap ) slot -= a.ebe_cap;
+                    }
+                }
                 if( lane_in( act ) && k >= 0 && k < a.O.n )
-                    store_record( a.O, k, P, i, L, C, pid, start + e, at_turn );
+                {
+                    if( narrow ) store_record_narrow( a.O, k, gamma0, L, C, pid, el, at_turn );
+                    else store_record( a.O, k, P, i, L, C, pid, el, at_turn );
+                }
             }
             uint64_t bad = 0u, failm = 0u;        // lost here / map failed
             if( op == ST_OP_DRIFT )
@@ -369,14 +418,14 @@ extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
 
 extern "C" hipError_t KN( st_launch_track_ebe )( const st_dev_particles_t* P,
     const uint64_t* lat, int64_t n_elem, int64_t until_turn,
-    const st_ebe_config_t* cfg, const st_dev_particles_t* O, int32_t* status,
+    const st_ebe_config_t* cfg, int64_t cap, const st_dev_particles_t* O, int32_t* status,
     int block, hipStream_t stream )
 {
     if( P->n <= 0 ) return hipSuccess;
     block = sane_block( block );
     Args a{};
     a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
-    a.cfg = *cfg; a.O = *O; a.status_out = status;
+    a.cfg = *cfg; a.ebe_cap = cap; a.O = *O; a.status_out = status;
     hipLaunchKernelGGL( KN( st_k_track_ebe ), dim3( grid_for( P->n, block ) ), dim3( block ),
                         0, stream, *P, a );
     return hipGetLastError();

@@ -11,7 +11,10 @@ benchmark definitions of this project):
   focusing/defocusing quads sized for a 72 deg FODO phase advance plus
   sextupoles and random orders, all with N(0, 1e-4) relative errors,
   2 XYShift/SRotation with N(0, 1e-4) offsets/angles, 1 LimitEllipse),
-  with 8 of the cells carrying a cavity in place of one drift.
+  with 8 of the cells carrying a cavity in place of one multipole.
+* ``collective_ring`` -- the 20,000-element ring plus 2 BeamBeam6D
+  interaction points (5 slices each), 1 BeamBeam4D and 16 SCQGaussProfile
+  kicks: 20,019 elements (element-by-element benchmark).
 * ``kitchen_sink``     -- short lattice containing every element type the
   trackers support (parity tests).
 * particle sets: Gaussian beams, polar amplitude grids.
@@ -115,7 +118,7 @@ def fodo_ring(n_cells=100, k1=0.01, k2=0.05, drift=1.0, voltage=12e6,
 def lhc_like_ring(seed=0, n_cells=400, tune=80.31, rel_err=1e-4, k2_scale=0.02,
                   r0=0.017, n_cavities=8, voltage=2e6, frequency=400e6,
                   ellipse_a=0.02, ellipse_b=0.015, mean_drift=5.05,
-                  error_seed=None, cbuffer=None, return_info=False):
+                  error_seed=None, cbuffer=None, return_info=False, after_cell=None):
     """20,000-element synthetic ring (module docstring).
 
     Each 50-element cell is [QF | 13 drifts, 9 multipoles, 1 shift/rotation
@@ -131,7 +134,8 @@ def lhc_like_ring(seed=0, n_cells=400, tune=80.31, rel_err=1e-4, k2_scale=0.02,
     errors on all its coefficients (normalised to the quad strength at the
     reference radius ``r0``).  ``seed`` fixes the structure, ``error_seed``
     (default ``seed``) the error table.  8 cells carry a cavity in place of
-    one multipole."""
+    one multipole.  ``after_cell(eb, c)``, when given, may append elements
+    after cell ``c``."""
     if error_seed is None:
         error_seed = seed
     rng = np.random.default_rng(seed)
@@ -197,6 +201,8 @@ def lhc_like_ring(seed=0, n_cells=400, tune=80.31, rel_err=1e-4, k2_scale=0.02,
                     misc_toggle ^= 1
         be.LimitEllipse(cbuffer=eb, a=ellipse_a, b=ellipse_b)
         counts["ellipse"] += 1
+        if after_cell is not None:
+            after_cell(eb, c)
     if return_info:
         return eb, {"counts": counts, "orders": orders, "k1": k1, "mu": mu,
                     "half_cells": (l1, l2), "mp_design": mp_design}
@@ -284,6 +290,81 @@ def adjoint_sublattice(seed=0, n_cells=20, tune=4.31, rel_err=1e-4, k2_scale=0.0
                 ksl = ksl + scale * erng.normal(0.0, 1.0, order + 1)
                 be.Multipole(cbuffer=eb, knl=knl, ksl=ksl)
     return eb
+
+
+def collective_ring(seed=0, bunch_charge=1.15e11, sigma_ip=16.7e-6, beta_ip=0.55,
+                    sigma_z=0.075, phi=142.5e-6, n_slices=5, n_sc=16, beta_arc=236.0,
+                    sc_sigma_ratio=0.55, n_cells=400):
+    """The 20,000-element ring with collective elements (20,019 elements;
+    ``n_cells`` < 400 gives a shorter ring of the same make-up):
+
+    * BeamBeam6D after cells 0 and n_cells / 2 (horizontal and vertical crossing,
+      half angle ``phi``): a round Gaussian opposing bunch of
+      ``bunch_charge`` charges, sigma_x = sigma_y = ``sigma_ip``,
+      sigma_px = sigma_py = sigma_ip / beta_ip, cut into ``n_slices``
+      equal-charge slices at the slice centroids of a Gaussian of rms
+      ``sigma_z``;
+    * BeamBeam4D after cell n_cells / 4 with the same charge and sizes;
+    * ``n_sc`` SCQGaussProfile kicks (q = 1, bunch length ``sigma_z``),
+      one after every (n_cells / n_sc)-th cell, each covering 1 / n_sc of the
+      circumference, with the arc beam size sigma = twiss_sigma(beta_arc)
+      horizontally and ``sc_sigma_ratio`` times it vertically (an
+      elliptical beam, so the kicks use the Faddeeva-function field).
+    """
+    from statistics import NormalDist
+    nd = NormalDist()
+    edges = [nd.inv_cdf(k / n_slices) if 0 < k < n_slices else None
+             for k in range(n_slices + 1)]
+    cent = []
+    for k in range(n_slices):
+        pa = nd.pdf(edges[k]) if edges[k] is not None else 0.0
+        pb = nd.pdf(edges[k + 1]) if edges[k + 1] is not None else 0.0
+        cent.append(sigma_z * n_slices * (pa - pb))
+    sig_ip2 = sigma_ip * sigma_ip
+    sig_p2 = (sigma_ip / beta_ip) ** 2
+    sig_arc = twiss_sigma(beta=beta_arc)
+    circumference = n_cells * 27 * 5.05
+    every = max(1, n_cells // n_sc)
+
+    def bb6d(eb, alpha):
+        be.BeamBeam6D(cbuffer=eb, phi=phi, alpha=alpha, x_bb_co=0.0, y_bb_co=0.0,
+                      charge_slices=[bunch_charge / n_slices] * n_slices,
+                      zeta_slices=cent, sigma_11=sig_ip2, sigma_12=0.0, sigma_13=0.0,
+                      sigma_14=0.0, sigma_22=sig_p2, sigma_23=0.0, sigma_24=0.0,
+                      sigma_33=sig_ip2, sigma_34=0.0, sigma_44=sig_p2, x_co=0.0,
+                      px_co=0.0, y_co=0.0, py_co=0.0, zeta_co=0.0, delta_co=0.0,
+                      min_sigma_diff=1e-10, threshold_singular=1e-28, d_x=0.0, d_px=0.0,
+                      d_y=0.0, d_py=0.0, d_zeta=0.0, d_delta=0.0, enabled=1)
+
+    def extra(eb, c):
+        if c == 0:
+            bb6d(eb, 0.0)
+        elif c == n_cells // 2:
+            bb6d(eb, 0.5 * math.pi)
+        elif c == n_cells // 4:
+            be.BeamBeam4D(cbuffer=eb, charge=bunch_charge, sigma_x=sigma_ip, sigma_y=sigma_ip,
+                          beta_r=1.0, min_sigma_diff=1e-10, x_bb=0.0, y_bb=0.0, d_px=0.0,
+                          d_py=0.0, enabled=1.0)
+        if c % every == every // 2 and c // every < n_sc:
+            be.SCQGaussProfile(cbuffer=eb, number_of_particles=bunch_charge,
+                               bunchlength_rms=sigma_z, sigma_x=sig_arc,
+                               sigma_y=sc_sigma_ratio * sig_arc,
+                               length=circumference / n_sc, q_param=1.0)
+
+    n_elem = 50 * n_cells
+    eb = CBuffer(max_slots=n_elem * 16 + 4096, max_objects=n_elem + 64, max_pointers=n_elem + 64)
+    # same phase advance per cell as the full ring
+    return lhc_like_ring(seed=seed, n_cells=n_cells, tune=80.31 * n_cells / 400.0,
+                         n_cavities=min(8, n_cells), cbuffer=eb, after_cell=extra)
+
+
+def matched_gaussian_beam(n, seed=0, beta=236.0, sigma_zeta=0.075, sigma_delta=1.1e-4):
+    """6D Gaussian beam matched to the ring start (alpha = 0, beta
+    ``beta``): sigma_x = sigma_y = twiss_sigma(beta), sigma_px = sigma_x /
+    beta; longitudinal rms ``sigma_zeta`` and ``sigma_delta``."""
+    s = twiss_sigma(beta=beta)
+    return gaussian_beam(n, seed=seed, sigma_x=s, sigma_px=s / beta, sigma_y=s,
+                         sigma_py=s / beta, sigma_zeta=sigma_zeta, sigma_delta=sigma_delta)
 
 
 def batch_workload(n_seeds=64, seed=0, min_particles=1000, max_particles=100000,

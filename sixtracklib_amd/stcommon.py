@@ -141,7 +141,7 @@ _fn("st_TrackJob_get_output_buffer", st_Buffer_p, [st_TrackJob_p])
 _fn("st_TrackJob_last_status", st_track_status_t, [st_TrackJob_p])
 _fn("st_TrackJob_required_output_sets", ct.c_int64,
     [st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p, st_buffer_size_t,
-     ct.POINTER(ct.c_int64), ct.c_int64, ct.POINTER(ct.c_int64)])
+     ct.POINTER(ct.c_int64), ct.c_int64, ct.POINTER(ct.c_int64), ct.c_char_p])
 _fn("st_TrackJob_last_kernel_time_ms", ct.c_double, [st_TrackJob_p])
 _fn("st_TrackJob_uses_fma", ct.c_int, [st_TrackJob_p])
 _fn("st_TrackJob_uses_full_kernel", ct.c_int, [st_TrackJob_p])
@@ -160,6 +160,58 @@ _fn("st_TrackBatch_autotune", ct.c_int, [st_TrackBatch_p, ct.c_int64])
 _fn("st_TrackBatch_num_seeds", ct.c_int64, [st_TrackBatch_p])
 _fn("st_TrackBatch_total_particles", ct.c_int64, [st_TrackBatch_p])
 _fn("st_TrackBatch_last_kernel_time_ms", ct.c_double, [st_TrackBatch_p])
+
+# -- stored buffers and address assignments (track_job.h:500-772) ----------
+st_arch_size_t = ct.c_uint64
+st_arch_status_t = ct.c_int32
+st_object_type_id_t = ct.c_uint64
+st_ARCH_STATUS_SUCCESS = ct.c_int32(0)
+st_ARCH_STATUS_GENERAL_FAILURE = ct.c_int32(-1)
+st_ARCH_ILLEGAL_BUFFER_ID = ct.c_uint64(0)
+st_ARCH_PARTICLES_BUFFER_ID = ct.c_uint64(1)
+st_ARCH_BEAM_ELEMENTS_BUFFER_ID = ct.c_uint64(2)
+st_ARCH_OUTPUT_BUFFER_ID = ct.c_uint64(3)
+st_ARCH_MIN_USER_DEFINED_BUFFER_ID = ct.c_uint64(1024)
+
+
+class st_AssignAddressItem(ct.Structure):
+    """buffer/assign_address_item.h:34-45"""
+    _fields_ = [(n, ct.c_uint64) for n in (
+        "dest_buffer_id", "src_buffer_id", "dest_elem_type_id", "src_elem_type_id",
+        "dest_elem_index", "src_elem_index", "dest_pointer_offset", "src_pointer_offset")]
+
+
+st_AssignAddressItem_p = ct.POINTER(st_AssignAddressItem)
+st_NullAssignAddressItem = None
+
+_fn("st_TrackJob_add_stored_buffer", st_arch_size_t,
+    [st_TrackJob_p, st_Buffer_p, ct.c_bool, ct.c_bool])
+_fn("st_TrackJob_create_stored_buffer", st_arch_size_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_remove_stored_buffer", st_arch_status_t, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_has_stored_buffers", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_num_stored_buffers", st_arch_size_t, [st_TrackJob_p])
+_fn("st_TrackJob_min_stored_buffer_id", st_arch_size_t, [st_TrackJob_p])
+_fn("st_TrackJob_max_stored_buffer_id", st_arch_size_t, [st_TrackJob_p])
+_fn("st_TrackJob_owns_stored_buffer", ct.c_bool, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_stored_buffer", st_Buffer_p, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_push_stored_buffer", st_arch_status_t, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_collect_stored_buffer", st_arch_status_t, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_add_assign_address_item", st_AssignAddressItem_p,
+    [st_TrackJob_p, st_AssignAddressItem_p])
+_fn("st_TrackJob_add_assign_address_item_detailed", st_AssignAddressItem_p,
+    [st_TrackJob_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJob_has_assign_items", ct.c_bool, [st_TrackJob_p, st_arch_size_t, st_arch_size_t])
+_fn("st_TrackJob_num_assign_items", st_arch_size_t,
+    [st_TrackJob_p, st_arch_size_t, st_arch_size_t])
+_fn("st_TrackJob_total_num_assign_items", st_arch_size_t, [st_TrackJob_p])
+_fn("st_TrackJob_commit_address_assignments", st_arch_status_t, [st_TrackJob_p])
+_fn("st_TrackJob_assign_all_addresses", st_arch_status_t, [st_TrackJob_p])
+_fn("st_TrackJob_assign_addresses", st_arch_status_t,
+    [st_TrackJob_p, st_buffer_size_t, st_buffer_size_t])
+_fn("st_Lattice_pack_with_tables", ct.c_int64,
+    [st_uchar_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
+     ct.c_void_p, ct.c_void_p])
 
 _fn("st_Hip_num_devices", ct.c_int, [])
 _fn("st_Hip_last_error_str", ct.c_char_p, [])

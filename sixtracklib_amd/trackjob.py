@@ -75,7 +75,8 @@ class TrackJob:
         has_ebe = ct.c_int64(0)
         n = st.st_TrackJob_required_output_sets(
             self._c_pbuf, len(particle_set_indices), idx, self._c_ebuf,
-            self._until_turn_ebe, sizes, 4096, ct.byref(has_ebe))
+            self._until_turn_ebe, sizes, 4096, ct.byref(has_ebe),
+            st.string_to_encoded_ctypes_str(cfg))
         if n < 0:
             raise RuntimeError("cannot determine the output requirements: " + st.last_error())
         if output_buffer is not None:
@@ -239,3 +240,142 @@ class TrackJob:
 
     def _push_output(self):
         st.st_TrackJob_push_output(self.ptr_st_track_job)
+
+    # -- stored buffers (python/sixtracklib/trackjob.py:2200-2285) ---------
+    @property
+    def has_stored_buffers(self):
+        return bool(st.st_TrackJob_has_stored_buffers(self.ptr_st_track_job))
+
+    @property
+    def num_stored_buffers(self):
+        return int(st.st_TrackJob_num_stored_buffers(self.ptr_st_track_job))
+
+    @property
+    def min_stored_buffer_id(self):
+        return int(st.st_TrackJob_min_stored_buffer_id(self.ptr_st_track_job))
+
+    @property
+    def max_stored_buffer_id(self):
+        return int(st.st_TrackJob_max_stored_buffer_id(self.ptr_st_track_job))
+
+    def add_stored_buffer(self, cbuffer=None, size=None, ptr_c_buffer=None,
+                          take_ownership=False, delete_ptr_after_move=False):
+        """Keep ``cbuffer`` (a CBuffer, mapped without a copy), a new empty
+        buffer of ``size`` bytes owned by the job, or an existing
+        ``NS(Buffer)*``; returns its buffer id (st_ARCH_ILLEGAL_BUFFER_ID
+        on failure)."""
+        if not hasattr(self, "_stored"):
+            self._stored = {}
+        bid = st.st_ARCH_ILLEGAL_BUFFER_ID.value
+        if cbuffer is not None:
+            cb = get_cbuffer_from_obj(cbuffer)
+            bid = st.st_TrackJob_add_stored_buffer(self.ptr_st_track_job, self._map(cb),
+                                                   False, False)
+            if bid != st.st_ARCH_ILLEGAL_BUFFER_ID.value:
+                self._stored[bid] = cb
+        elif size is not None and size > 0:
+            bid = st.st_TrackJob_create_stored_buffer(self.ptr_st_track_job, int(size))
+        elif ptr_c_buffer is not None:
+            bid = st.st_TrackJob_add_stored_buffer(self.ptr_st_track_job, ptr_c_buffer,
+                                                   bool(take_ownership),
+                                                   bool(delete_ptr_after_move))
+        return int(bid)
+
+    def remove_stored_buffer(self, buffer_id):
+        getattr(self, "_stored", {}).pop(buffer_id, None)
+        return st.st_TrackJob_remove_stored_buffer(self.ptr_st_track_job, int(buffer_id))
+
+    def owns_stored_buffer(self, buffer_id):
+        return bool(st.st_TrackJob_owns_stored_buffer(self.ptr_st_track_job, int(buffer_id)))
+
+    def stored_buffer_is_cbuffer(self, buffer_id):
+        return buffer_id in getattr(self, "_stored", {})
+
+    def stored_buffer_cbuffer(self, buffer_id):
+        if not self.stored_buffer_is_cbuffer(buffer_id):
+            raise RuntimeError("Unable to retrieve CBuffer for buffer_id=%d" % buffer_id)
+        return self._stored[buffer_id]
+
+    def ptr_stored_buffer(self, buffer_id):
+        return st.st_TrackJob_stored_buffer(self.ptr_st_track_job, int(buffer_id))
+
+    def stored_buffer(self, buffer_id):
+        return getattr(self, "_stored", {}).get(buffer_id, None)
+
+    def push_stored_buffer(self, buffer_id):
+        self._last_status = st.st_TrackJob_push_stored_buffer(self.ptr_st_track_job,
+                                                              int(buffer_id))
+        if self._last_status != 0:
+            raise RuntimeError("Unable to push stored buffer %d" % buffer_id)
+        return self
+
+    def collect_stored_buffer(self, buffer_id):
+        self._last_status = st.st_TrackJob_collect_stored_buffer(self.ptr_st_track_job,
+                                                                 int(buffer_id))
+        if self._last_status != 0:
+            raise RuntimeError("Unable to collect stored buffer %d" % buffer_id)
+        return self
+
+    # -- address assignments (python/sixtracklib/trackjob.py:1833-2040) ----
+    def add_assign_address_item(self, item=None, dest_elem_type_id=None, dest_buffer_id=None,
+                                dest_elem_index=None, dest_pointer_offset=None,
+                                src_elem_type_id=None, src_buffer_id=None, src_elem_index=None,
+                                src_pointer_offset=None):
+        ptr = None
+        if item is not None:
+            if isinstance(item, st.st_AssignAddressItem):
+                ptr = st.st_TrackJob_add_assign_address_item(self.ptr_st_track_job,
+                                                             ct.byref(item))
+            else:                                  # an AssignAddressItem CObject
+                it = st.st_AssignAddressItem(*[int(getattr(item, f)) for f, _ in
+                                               st.st_AssignAddressItem._fields_])
+                ptr = st.st_TrackJob_add_assign_address_item(self.ptr_st_track_job,
+                                                             ct.byref(it))
+        elif None not in (dest_elem_type_id, dest_buffer_id, dest_elem_index,
+                          dest_pointer_offset, src_elem_type_id, src_buffer_id,
+                          src_elem_index, src_pointer_offset):
+            ptr = st.st_TrackJob_add_assign_address_item_detailed(
+                self.ptr_st_track_job, int(dest_elem_type_id), int(dest_buffer_id),
+                int(dest_elem_index), int(dest_pointer_offset), int(src_elem_type_id),
+                int(src_buffer_id), int(src_elem_index), int(src_pointer_offset))
+        if not ptr:
+            raise ValueError("unable to add AssignAddressItem given by these parameters")
+        return ptr
+
+    def has_assign_items(self, dest_buffer_id, src_buffer_id):
+        return bool(st.st_TrackJob_has_assign_items(self.ptr_st_track_job, int(dest_buffer_id),
+                                                    int(src_buffer_id)))
+
+    def num_assign_items(self, dest_buffer_id, src_buffer_id):
+        return int(st.st_TrackJob_num_assign_items(self.ptr_st_track_job, int(dest_buffer_id),
+                                                   int(src_buffer_id)))
+
+    @property
+    def total_num_assign_items(self):
+        return int(st.st_TrackJob_total_num_assign_items(self.ptr_st_track_job))
+
+    def commit_address_assignments(self):
+        self._last_status = st.st_TrackJob_commit_address_assignments(self.ptr_st_track_job)
+        if self._last_status != 0:
+            raise RuntimeError("Unable to commit address assignment items")
+        return self
+
+    def assign_all_addresses(self):
+        self._last_status = st.st_TrackJob_assign_all_addresses(self.ptr_st_track_job)
+        if self._last_status != 0:
+            raise RuntimeError("Unable to perform assignment of all address items: " +
+                               st.last_error())
+        return self
+
+    def assign_addresses(self, dest_buffer_id, src_buffer_id):
+        if dest_buffer_id is None and src_buffer_id is None:
+            return self.assign_all_addresses()
+        if dest_buffer_id is None or src_buffer_id is None:
+            raise ValueError("give both dest_buffer_id and src_buffer_id, or neither")
+        self._last_status = st.st_TrackJob_assign_addresses(self.ptr_st_track_job,
+                                                            int(dest_buffer_id),
+                                                            int(src_buffer_id))
+        if self._last_status != 0:
+            raise RuntimeError("Unable to perform assignment of address items: " +
+                               st.last_error())
+        return self

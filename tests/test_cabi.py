@@ -111,6 +111,80 @@ def test_lattice_pack_all_element_types():
     assert cav[2] == ((2.0 * np.pi) / 299792458.0) * 200e6
 
 
+def _sc_interpol_lattice(table_cb):
+    eb = CBuffer()
+    be.Drift(cbuffer=eb, length=1.0)
+    be.SCInterpolatedProfile(cbuffer=eb, number_of_particles=1e11, sigma_x=1e-3,
+                             sigma_y=1.2e-3, length=1.0, line_density_prof_fallback=0.5,
+                             interpol_data_addr=table_cb.get_object_address(0))
+    return eb
+
+
+def _pack(eb, tables=()):
+    from sixtracklib_amd import stcommon as st
+    arr = (ct.c_void_p * max(1, len(tables)))(*[t.pointer for t in tables])
+    n_el = ct.c_int64(0)
+    nw = st.st_Lattice_pack_with_tables(eb.pointer, arr, len(tables), None, 0, None,
+                                        ct.byref(n_el), None, None)
+    assert nw > 0
+    out = np.zeros(nw, dtype=np.uint64)
+    offs = np.zeros(n_el.value + 1, dtype=np.uint32)
+    assert st.st_Lattice_pack_with_tables(eb.pointer, arr, len(tables), out.ctypes.data, nw,
+                                          offs.ctypes.data, ct.byref(n_el), None, None) == nw
+    return out, offs
+
+
+def test_lattice_pack_copies_tables_from_table_buffers():
+    """An SCInterpolatedProfile whose table lies in another buffer: without
+    that buffer the record selects the fallback density (method 255); with
+    it the record carries method, size, z0, dz and both arrays."""
+    tb = CBuffer()
+    vals = np.linspace(1.0, 2.0, 9)
+    be.LineDensityProfileData(cbuffer=tb, method=0, capacity=9, num_values=9, z0=-0.4, dz=0.1,
+                              values=vals, derivatives=np.full(9, 10.0))
+    eb = _sc_interpol_lattice(tb)
+    out, offs = _pack(eb)
+    rec = out[offs[1]: offs[2]]
+    assert int(rec[2 + 10]) == 255 and int(rec[2 + 11]) == 0
+    out, offs = _pack(eb, [tb])
+    rec = out[offs[1]: offs[2]]
+    assert int(rec[2 + 10]) == 0 and int(rec[2 + 11]) == 9
+    assert rec[2 + 12: 2 + 14].view(np.float64).tolist() == [-0.4, 0.1]
+    assert np.array_equal(rec[2 + 14: 2 + 23].view(np.float64), vals)
+    assert np.all(rec[2 + 23: 2 + 32].view(np.float64) == 10.0)
+    # a missing table-buffer list is an error
+    from sixtracklib_amd import stcommon as st
+    assert st.st_Lattice_pack_with_tables(eb.pointer, None, 1, None, 0, None, None, None,
+                                          None) == -1
+
+
+def test_line_density_interpolation_preparation():
+    """Linear: forward differences; cubic: second derivatives of the natural
+    spline (scipy CubicSpline bc_type='natural' as the independent check);
+    interpol() evaluates both forms and is 0 outside the table."""
+    from scipy.interpolate import CubicSpline
+    z = -1.0 + 0.05 * np.arange(41)
+    y = np.exp(-z ** 2) * (1.0 + 0.3 * z)
+    cb = CBuffer()
+    lin = be.LineDensityProfileData(cbuffer=cb, method="linear", capacity=41, z0=-1.0, dz=0.05,
+                                    values=y)
+    cub = be.LineDensityProfileData(cbuffer=cb, method="cubic", capacity=41, z0=-1.0, dz=0.05,
+                                    values=y)
+    lin.prepare_interpolation()
+    cub.prepare_interpolation()
+    assert np.array_equal(np.array(lin.derivatives)[:40], (y[1:] - y[:-1]) / 0.05)
+    cs = CubicSpline(z, y, bc_type="natural")
+    np.testing.assert_allclose(cub.derivatives, cs(z, 2), rtol=0, atol=1e-12)
+    for zz in (-0.93, -0.2, 0.0, 0.4441, 0.97):
+        assert abs(cub.interpol(zz) - cs(zz)) < 1e-14
+        i = int((zz + 1.0) / 0.05)
+        assert lin.interpol(zz) == y[i] + (zz - (-1.0 + 0.05 * i)) * lin.derivatives[i]
+    assert lin.interpol(-1.5) == 0.0 and cub.interpol(1.01) == 0.0
+    with pytest.raises(RuntimeError):
+        be.LineDensityProfileData(cbuffer=cb, method="cubic", capacity=3, values=[1, 2, 3]
+                                  ).prepare_interpolation()
+
+
 def test_lattice_pack_rejects_unrelocated_buffer():
     from sixtracklib_amd import stcommon as st
     eb = lattices.fodo_ring(n_cells=2)
@@ -136,10 +210,17 @@ def test_trackjob_required_output_sets():
     ce = st.st_Buffer_new_on_memory(eb.pointer, eb.words.nbytes)
     sizes = (ct.c_int64 * 8)()
     has = ct.c_int64(0)
-    n = st.st_TrackJob_required_output_sets(cp, 0, None, ce, 3, sizes, 8, ct.byref(has))
+    n = st.st_TrackJob_required_output_sets(cp, 0, None, ce, 3, sizes, 8, ct.byref(has), None)
     assert n == 2 and has.value == 1
     assert sizes[0] == 10 * eb.n_objects * 3
     assert sizes[1] == 4 * 10
+    # a store of 5 element passes per particle; a larger request keeps the full size
+    n = st.st_TrackJob_required_output_sets(cp, 0, None, ce, 3, sizes, 8, ct.byref(has),
+                                            b"fma=1 ebe_store_passes=5")
+    assert n == 2 and sizes[0] == 10 * 5 and sizes[1] == 4 * 10
+    n = st.st_TrackJob_required_output_sets(cp, 0, None, ce, 3, sizes, 8, ct.byref(has),
+                                            b"ebe_store_passes=100000")
+    assert sizes[0] == 10 * eb.n_objects * 3
     st.st_Buffer_delete(cp)
     st.st_Buffer_delete(ce)
 
