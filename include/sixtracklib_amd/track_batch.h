@@ -22,8 +22,12 @@
  *                         lattice shared in that XCD's L2), 1: one queue
  *   turns_per_launch=K    turns per chunk (0: from the work size)
  *   graph=0|1             capture the chunks of a call in a HIP graph (1)
- * NS(TrackBatch_autotune) picks block, waves_per_cu and queues by timing
- * short trials on the batch itself (state restored afterwards).
+ *   ppl=1|2               particles per lane (1): with 2 every lane tracks
+ *                         two particles through the shared record walk
+ *                         (128-particle tiles; results identical)
+ * NS(TrackBatch_autotune) picks particles per lane, block, waves_per_cu
+ * and queues by timing short trials on the batch itself (state restored
+ * afterwards).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_BATCH_H__
 #define SIXTRACKLIB_AMD_TRACK_BATCH_H__

@@ -26,7 +26,9 @@
  * "turns_per_launch=K" (default 0: chosen from the work size; with
  * compact=1 a request is split into launches of K turns and the particle
  * list is rebuilt between them; results identical), "queues=1|8" (default
- * 1: one work queue for the persistent launch; 8: one per XCD),
+ * 1: one work queue for the persistent launch; 8: one per XCD), "ppl=1|2"
+ * (default 1: particles per lane of the until-turn kernel of lean
+ * lattices; results identical),
  * "ebe_store_passes=K" (default 0: the element-by-element store holds
  * particles x elements x turns records, as the reference's; K > 0 sizes it
  * for K element passes of every particle when that is fewer, and with

@@ -205,9 +205,12 @@ struct NS(TrackBatch)
         return *h_status;
     }
 
-    // Candidate (block, waves_per_cu, queues) timed on trial_turns turns of
-    // the batch itself; the particle images are saved first and restored
-    // after every trial, so the batch state is unchanged.
+    // Candidate (particles per lane, block, waves_per_cu, queues) timed on
+    // trial_turns turns of the batch itself; the particle images are saved
+    // first and restored after every trial, so the batch state is
+    // unchanged.  One particle per lane is tried at 4 and 8 resident waves
+    // per SIMD, two particles per lane at 2.5 and 5 (their register
+    // budgets allow 8 and 5).
     int autotune( int64_t trial_turns )
     {
         if( trial_turns <= 0 ) trial_turns = 2;
@@ -221,14 +224,16 @@ struct NS(TrackBatch)
         Config best = base;
         float best_ms = -1.0f;
         int const blocks[] = { 64, 128, 256 };
-        int const waves[] = { 16, 32 };
+        int const waves[ 2 ][ 2 ] = { { 16, 32 }, { 10, 20 } };
         int const queues[] = { 1, 8 };
+        for( int ppl = 1; ppl <= 2; ++ppl )
         for( int b : blocks )
-            for( int w : waves )
+            for( int w : waves[ ppl - 1 ] )
                 for( int q : queues )
                 {
                     if( !ok ) break;
                     run.cfg = base;
+                    run.cfg.ppl = ppl;
                     run.cfg.block = b; run.cfg.waves_per_cu = w; run.cfg.queues = q;
                     run.cfg.graph = false;
                     if( !run.build_queues( stream ) ) { ok = false; break; }

@@ -132,6 +132,47 @@ KN( st_k_batch_track )( st_batch_sched_t s, int64_t until_turn, int64_t max_turn
     }
 }
 
+// Occupancy target of the two-particles-per-lane track kernel: its two
+// state sets fit 96 VGPRs, 5 waves per SIMD.
+#ifndef ST_LEAN2_WAVES_PER_EU
+#define ST_LEAN2_WAVES_PER_EU 5
+#endif
+
+// As st_k_batch_track with 128-particle tiles, two particles per lane.
+__global__ void __launch_bounds__( 256, ST_LEAN2_WAVES_PER_EU )
+KN( st_k_batch_track2 )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
+                         int32_t* status )
+{
+    ci32_t* const counts = ( ci32_t* )s.counts;
+    ci32_t* const qseeds = ( ci32_t* )s.queue_seeds;
+    ci32_t* const qbegin = ( ci32_t* )s.queue_begin;
+    int const nq = s.n_queues;
+    int const q0 = ( int )( blockIdx.x % ( unsigned )nq );
+    for( int qi = 0; qi < nq; ++qi )
+    {
+        int const q = ( q0 + qi ) % nq;
+        int const qb = qbegin[ q ], qe = qbegin[ q + 1 ];
+        for( ;; )
+        {
+            int t = 0;
+            if( __lane_id() == 0 ) t = atomicAdd( &s.queue_next[ q ], 1 );
+            t = __builtin_amdgcn_readfirstlane( __shfl( t, 0 ) );
+            int seed = -1;
+            for( int k = qb; k < qe; ++k )
+            {
+                int const sd = qseeds[ k ];
+                int const nt = ( counts[ 2 * sd ] + 127 ) >> 7;
+                if( t < nt ) { seed = sd; break; }
+                t -= nt;
+            }
+            if( seed < 0 ) break;                    // this queue is drained
+            SeedView const S = load_seed( s.seeds, seed );
+            track_lean_lane2( S.P, S.list, counts[ 2 * seed ], ( int64_t )t * 128, S.lat,
+                              S.n_elem, until_turn, max_turns, status );
+        }
+    }
+}
+
 // ------------------------------------------------------------ launchers
 
 extern "C" hipError_t KN( st_launch_batch_reset )( const st_batch_sched_t* s, hipStream_t stream )
@@ -152,12 +193,16 @@ extern "C" hipError_t KN( st_launch_batch_select )( const st_batch_sched_t* s,
 
 extern "C" hipError_t KN( st_launch_batch_track )( const st_batch_sched_t* s,
     int64_t until_turn, int64_t max_turns, int32_t* status, int block, int grid,
-    hipStream_t stream )
+    int particles_per_lane, hipStream_t stream )
 {
     if( s->n_seeds <= 0 || s->n_queues <= 0 ) return hipSuccess;
     if( block <= 0 || block > 256 || ( block % 64 ) != 0 ) block = 256;
     if( grid <= 0 ) grid = 1;
-    hipLaunchKernelGGL( KN( st_k_batch_track ), dim3( ( unsigned )grid ), dim3( ( unsigned )block ),
-                        0, stream, *s, until_turn, max_turns, status );
+    if( particles_per_lane == 2 )
+        hipLaunchKernelGGL( KN( st_k_batch_track2 ), dim3( ( unsigned )grid ),
+                            dim3( ( unsigned )block ), 0, stream, *s, until_turn, max_turns, status );
+    else
+        hipLaunchKernelGGL( KN( st_k_batch_track ), dim3( ( unsigned )grid ),
+                            dim3( ( unsigned )block ), 0, stream, *s, until_turn, max_turns, status );
     return hipGetLastError();
 }

@@ -51,13 +51,14 @@ struct Config
     int queues = 8;
     int64_t turns_per_launch = 0;
     bool graph = true;
+    int ppl = 1;                 // particles per lane (1 or 2)
 
     std::string str() const
     {
-        char b[ 160 ];
+        char b[ 192 ];
         std::snprintf( b, sizeof( b ), "fma=%d block=%d waves_per_cu=%d queues=%d "
-                       "turns_per_launch=%lld graph=%d", fma ? 1 : 0, block, waves_per_cu,
-                       queues, ( long long )turns_per_launch, graph ? 1 : 0 );
+                       "turns_per_launch=%lld graph=%d ppl=%d", fma ? 1 : 0, block, waves_per_cu,
+                       queues, ( long long )turns_per_launch, graph ? 1 : 0, ppl );
         return b;
     }
 
@@ -83,6 +84,11 @@ struct Config
         }
         else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
         else if( k == "graph" ) graph = iv != 0;
+        else if( k == "ppl" )
+        {
+            if( iv != 1 && iv != 2 ) return fail( "ppl (particles per lane) must be 1 or 2" );
+            ppl = ( int )iv;
+        }
         return true;
     }
 
@@ -269,8 +275,10 @@ struct BatchRunner
         {
             if( !enqueue_reset_select( until_turn, stream ) ) return false;
             hipError_t const e = cfg.fma
-                ? st_launch_batch_track_fma( &s, until_turn, k, d_status, cfg.block, grid(), stream )
-                : st_launch_batch_track_exact( &s, until_turn, k, d_status, cfg.block, grid(), stream );
+                ? st_launch_batch_track_fma( &s, until_turn, k, d_status, cfg.block, grid(), cfg.ppl,
+                                             stream )
+                : st_launch_batch_track_exact( &s, until_turn, k, d_status, cfg.block, grid(), cfg.ppl,
+                                               stream );
             if( !hip_ok( e, "launch batch track" ) ) return false;
         }
         return true;

@@ -177,6 +177,7 @@ struct NS(TrackJobBase)
     // one tile queue: a job's sets share one lattice, and with a queue per
     // XCD the waves of the first XCD would take every tile of a small set
     int queues = 1;
+    int ppl = 1;                         // particles per lane of the lean until path
     st_batch::BatchRunner runner;        // scheduling of the lean until path
     bool runner_ready = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -483,6 +484,10 @@ struct NS(TrackJobBase)
         runner.cfg.block = ( block == 64 || block == 128 ) ? block : 256;
         runner.cfg.turns_per_launch = turns_per_launch;
         runner.cfg.queues = queues;
+        runner.cfg.ppl = ppl;
+        // resident waves per CU: 8 per SIMD with one particle per lane, 5
+        // with two (the kernels' register budgets)
+        runner.cfg.waves_per_cu = ppl == 2 ? 20 : 32;
         runner.cfg.graph = false;
         std::vector< st_batch_seed_t > seeds;
         for( const st_dev_particles_t& v : views )
@@ -585,6 +590,7 @@ struct NS(TrackJobBase)
             else if( k == "compact" ) compact = iv != 0;
             else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
             else if( k == "queues" ) queues = ( iv == 8 ) ? 8 : 1;
+            else if( k == "ppl" ) ppl = ( iv == 2 ) ? 2 : 1;
         }
     }
 };

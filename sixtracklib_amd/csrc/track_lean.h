@@ -175,6 +175,138 @@ ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, i
     if( in_range && status != 0 ) atomicOr( status_out, status );
 }
 
+// Two particles per lane: lane l of a wave tracks the particles of slots
+// base + l and base + 64 + l (list entries, or indices without a list)
+// through the same records.  Each particle's map sequence, loss handling
+// and stored values are exactly those of track_lean_lane; what the pair
+// shares is the record walk (scalar loads, header tests, branches), and
+// the two state sets give the scheduler two independent dependency chains
+// per lane (the Horner recurrences in particular).
+ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, int64_t n_lanes,
+                              int64_t base, cptr_t lat, int64_t n_elem, int64_t until_turn,
+                              int64_t max_turns, int32_t* status_out )
+{
+    constexpr int K = 2;
+    int64_t i[ K ], at_turn[ K ], start[ K ], turn_stop[ K ];
+    double chi[ K ];
+    Lane L[ K ];
+    uint64_t act[ K ];
+    int32_t status = 0;
+#pragma unroll
+    for( int k = 0; k < K; ++k )
+    {
+        int64_t const tid = base + 64 * k + __lane_id();
+        bool const in_range = tid < n_lanes;
+        i[ k ] = ( list != nullptr && in_range ) ? ( int64_t )list[ tid ] : tid;
+        int64_t const li = in_range ? i[ k ] : 0;
+        L[ k ].x = P.x[ li ]; L[ k ].px = P.px[ li ]; L[ k ].y = P.y[ li ];
+        L[ k ].py = P.py[ li ]; L[ k ].zeta = P.zeta[ li ]; L[ k ].delta = P.delta[ li ];
+        L[ k ].rpp = P.rpp[ li ]; L[ k ].rvv = P.rvv[ li ]; L[ k ].psigma = P.psigma[ li ];
+        L[ k ].s = P.s[ li ];
+        chi[ k ] = P.chi[ li ];
+        start[ k ] = P.at_element[ li ];
+        at_turn[ k ] = P.at_turn[ li ];
+        bool const active0 = in_range && n_elem > 0 && until_turn > at_turn[ k ] &&
+                             P.state[ li ] == 1;
+        if( in_range && n_elem <= 0 ) status = -1;
+        turn_stop[ k ] = ( max_turns > 0 && until_turn - at_turn[ k ] > max_turns )
+                             ? at_turn[ k ] + max_turns : until_turn;
+        act[ k ] = ballot( active0 );
+    }
+
+    while( ( act[ 0 ] | act[ 1 ] ) != 0u )
+    {
+        uint32_t off = 0;
+        uint64_t h = rec_at( lat, 0 )[ 0 ];
+        for( ;; )
+        {
+#if defined( ST_FMA_BUILD )
+            double rvvm1[ K ];
+#pragma unroll
+            for( int k = 0; k < K; ++k ) rvvm1[ k ] = L[ k ].rvv - 1.0;
+#endif
+            while( ( ( uint32_t )h & ST_OP_RARE_BIT ) == 0u )   // Drift / Multipole run
+            {
+                cptr_t const r = rec_at( lat, off );
+                uint32_t const noff = off + ST_REC_BYTES( h );
+                uint64_t const hn = rec_at( lat, noff )[ 0 ];
+                if( ( uint32_t )h & ST_OP_MP_BIT )
+                {
+#pragma unroll
+                    for( int k = 0; k < K; ++k )
+                        multipole( L[ k ], chi[ k ], r + ST_REC_PAYLOAD, ST_REC_AUX( h ),
+                                   L[ k ].delta );
+                }
+                else
+                {
+                    double const length = rd( r, ST_REC_PAYLOAD );
+#pragma unroll
+                    for( int k = 0; k < K; ++k )
+                    {
+#if defined( ST_FMA_BUILD )
+                        drift_rvvm1( L[ k ], length, rvvm1[ k ] );
+#else
+                        drift( L[ k ], length );
+#endif
+                    }
+#pragma unroll
+                    for( int k = 0; k < K; ++k )
+                    {
+                        uint64_t const lost = limit_global_lost_mask( L[ k ] ) & act[ k ];
+                        if( lost != 0u )
+                        {
+                            if( lane_in( lost ) )
+                                store_lane_state( P, i[ k ], L[ k ], 0, start[ k ] + ri( r, 1 ),
+                                                  at_turn[ k ] );
+                            act[ k ] &= ~lost;
+                        }
+                    }
+                }
+                off = noff;
+                h = hn;
+            }
+            unsigned const op = ST_REC_OP( h );
+            if( op == ST_OP_END ) break;
+            cptr_t const r = rec_at( lat, off );
+            uint32_t const noff = off + ST_REC_BYTES( h );
+            uint64_t const hn = rec_at( lat, noff )[ 0 ];
+#pragma unroll
+            for( int k = 0; k < K; ++k )
+            {
+                int64_t const li = lane_in( act[ k ] ) ? i[ k ] : 0;
+                bool fail = false;
+                bool const ok = lean_other( op, r + ST_REC_PAYLOAD, L[ k ], P, li, fail );
+                uint64_t const lost = ballot( !ok ) & act[ k ];
+                if( lost != 0u )
+                {
+                    if( lane_in( lost ) )
+                    {
+                        store_lane_state( P, i[ k ], L[ k ], 0, start[ k ] + ri( r, 1 ),
+                                          at_turn[ k ] );
+                        if( fail ) status = -1;
+                    }
+                    act[ k ] &= ~lost;
+                }
+            }
+            off = noff;
+            h = hn;
+        }
+#pragma unroll
+        for( int k = 0; k < K; ++k )
+        {
+            if( lane_in( act[ k ] ) ) { ++at_turn[ k ]; L[ k ].s = 0.0; }
+            uint64_t const fin = ballot( at_turn[ k ] >= turn_stop[ k ] ) & act[ k ];
+            if( fin != 0u )
+            {
+                if( lane_in( fin ) ) store_lane_state( P, i[ k ], L[ k ], 1, start[ k ],
+                                                       at_turn[ k ] );
+                act[ k ] &= ~fin;
+            }
+        }
+    }
+    if( status != 0 ) atomicOr( status_out, status );
+}
+
 // Indices of the particles a track_until call has work for (state 1 and
 // at_turn < until_turn) appended, wave by wave, to ``list``: one atomic add
 // per wave reserves the wave's slots, so the list is in no particular

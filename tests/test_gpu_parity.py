@@ -146,8 +146,10 @@ def test_fodo_ring_losses_in_steps_and_compaction():
     oracle_lib.track_until(ob, eb, 10)
     from sixtracklib_amd.trackjob import TrackJob
     got = {}
+    pb4 = _copy(pb)
     for key, buf, cfg in (("on", pb, "fma=0"), ("off", pb2, "fma=0 compact=0"),
-                          ("chunked", pb3, "fma=0 turns_per_launch=2")):
+                          ("chunked", pb3, "fma=0 turns_per_launch=2"),
+                          ("pairs", pb4, "fma=0 ppl=2 turns_per_launch=3")):
         job = TrackJob(eb, buf, config_str=cfg)
         for t in (3, 7, 10):
             assert job.track_until(t) == 0
@@ -158,6 +160,34 @@ def test_fodo_ring_losses_in_steps_and_compaction():
     assert_particles_equal(got["on"], want, what="stepped, gathered")
     assert_particles_equal(got["off"], want, what="stepped, not gathered")
     assert_particles_equal(got["chunked"], want, what="stepped, 2 turns per launch")
+    assert_particles_equal(got["pairs"], want, what="stepped, two particles per lane")
+
+
+@pytest.mark.parametrize("fma", [False, True])
+def test_two_particles_per_lane_match_one(fma):
+    """The two-particles-per-lane kernel gives every particle the result of
+    the one-particle kernel bit for bit (exact and FMA builds), on a ring
+    with losses, a particle count that leaves the last tile part-filled,
+    and a Cavity, XYShift and SRotation in the lean lattice."""
+    from sixtracklib_amd.trackjob import TrackJob
+    eb = lattices.lhc_like_ring(seed=3, n_cells=8, tune=8.31 * 8 / 40)
+    sig = lattices.twiss_sigma(beta=236.0)
+    pb, _ = lattices.polar_grid(45, 64, sigma_x=sig, sigma_y=sig, max_sigma=200.0,
+                                delta=2.7e-4)
+    pa = _copy(pb)
+    ob = _copy(pb)
+    res = {}
+    for key, buf, cfg in (("one", pa, "ppl=1"), ("two", pb, "ppl=2")):
+        job = TrackJob(eb, buf, config_str="fma=%d %s" % (int(fma), cfg))
+        for t in (4, 9, 20):
+            assert job.track_until(t) == 0
+        job.collect()
+        res[key] = _pdict(buf)
+    assert 50 < np.count_nonzero(res["one"]["state"] == 0) < 2800
+    assert_particles_equal(res["two"], res["one"], what="pairs vs single")
+    if not fma:
+        oracle_lib.track_until(ob, eb, 20)
+        assert_particles_equal(res["two"], _pdict(ob), what="pairs vs oracle")
 
 
 @pytest.mark.parametrize("seed", [0, 1])
