@@ -122,6 +122,7 @@ def batched_vs_sequential(n_seeds, turns):
     for j in jobs:
         j.track_until(turns)
     seq_wall = time.perf_counter() - t0
+    progress("sequential jobs: %.1f s" % seq_wall)
     for j in jobs:
         j.collect_particles()
     del jobs
@@ -253,6 +254,12 @@ def calibration_copy(nbytes):
     del a, b
 
 
+def progress(msg):
+    """One line per finished part of the run, on stderr (the JSON line is
+    the only stdout output)."""
+    print("bench.py: " + msg, file=sys.stderr, flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as f:
@@ -333,6 +340,7 @@ def main():
     if args.calibrate_copy_mib > 0:
         calibration_copy(args.calibrate_copy_mib << 20)
     tracked, wall, kernel_ms, alive = run(args.mode, args.steps, args.warmup)
+    progress("plain: %d particle-turns in %.2f s" % (tracked, wall))
     value = tracked / wall
     kernel_s = kernel_ms / 1e3
     achieved_tflops = tracked * flops_pt / kernel_s / 1e12
@@ -350,14 +358,17 @@ def main():
     batched = None
     if args.batch_seeds > 0:
         batched = batched_vs_sequential(args.batch_seeds, args.batch_turns)
+        progress("batched: %.3g particle-turns/s" % batched["value"])
 
     adjoint = None
     if not args.no_adjoint:
         adjoint = adjoint_cost()
+        progress("adjoint: cost ratio %.2f" % adjoint["exact"]["cost_ratio"])
 
     ebe = None
     if args.ebe_turns > 0:
         ebe = elem_by_elem_collective(args.ebe_particles, args.ebe_turns)
+        progress("element by element: %.0f GB/s" % ebe["roofline"]["achieved"])
 
     cpu = None
     if not args.no_cpu_baseline:
