@@ -220,6 +220,17 @@ def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
     job.collect_particles()
     at_turn, at_elem = np.array(p.at_turn), np.array(p.at_element)
     alive = np.array(p.state) == 1
+    # the same particles and turns through track_until (no records stored)
+    ebe_after = pb.words.copy()
+    pb.words[:] = init
+    job.push_particles()
+    t0 = time.perf_counter()
+    job.track_until(turns)
+    until_wall = time.perf_counter() - t0
+    job.collect_particles()
+    until_tracked = int(np.sum(np.array(p.at_turn)))
+    same_as_ebe = bool(np.array_equal(pb.words, ebe_after))
+    pb.words[:] = ebe_after
     # a record is stored before every element a particle enters, the one
     # where it is lost included
     records = int(np.sum(np.where(alive, at_turn * n_elem, at_turn * n_elem + at_elem + 1)))
@@ -235,6 +246,8 @@ def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
             "records": records, "alive_at_end": int(np.count_nonzero(alive)),
             "kernel_seconds": kms / 1e3, "wall_seconds": wall,
             "value": int(np.sum(at_turn)) / wall, "unit": "particle-turns/s",
+            "track_until_value": until_tracked / until_wall,
+            "track_until_same_particles_as_elem_by_elem": same_as_ebe,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": gbs / HBM_PEAK_GBS, "bytes_per_record": 168,
                          "traffic_per_turn": EBE_TURN_HBM_BYTES,

@@ -98,9 +98,11 @@ __global__ void __launch_bounds__( 256 ) KN( st_k_batch_select )( st_batch_sched
     select_active_lane( S.P, i, until_turn, S.list, &s.counts[ 2 * blockIdx.y ] );
 }
 
-__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
-KN( st_k_batch_track )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
-                        int32_t* status )
+// Tile loop of the persistent track kernels (HEAVY: lattices that also
+// hold the maps outside the lean set).
+template< bool HEAVY >
+ST_DEV void batch_track_tiles( const st_batch_sched_t& s, int64_t until_turn, int64_t max_turns,
+                               int32_t* status )
 {
     ci32_t* const counts = ( ci32_t* )s.counts;
     ci32_t* const qseeds = ( ci32_t* )s.queue_seeds;
@@ -126,10 +128,28 @@ KN( st_k_batch_track )( st_batch_sched_t s, int64_t until_turn, int64_t max_turn
             }
             if( seed < 0 ) break;                    // this queue is drained
             SeedView const S = load_seed( s.seeds, seed );
-            track_lean_lane( S.P, S.list, counts[ 2 * seed ], ( int64_t )t * 64 + __lane_id(),
-                             S.lat, S.n_elem, until_turn, max_turns, status );
+            track_lean_lane< HEAVY >( S.P, S.list, counts[ 2 * seed ],
+                                      ( int64_t )t * 64 + __lane_id(), S.lat, S.n_elem,
+                                      until_turn, max_turns, status );
         }
     }
+}
+
+__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
+KN( st_k_batch_track )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
+                        int32_t* status )
+{
+    batch_track_tiles< false >( s, until_turn, max_turns, status );
+}
+
+// The same for lattices with monitors, beam-beam, space charge or RF
+// multipoles: those records run out of line (heavy_step), the Drift /
+// Multipole runs keep the lean loop's registers and occupancy.
+__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
+KN( st_k_batch_track_heavy )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
+                              int32_t* status )
+{
+    batch_track_tiles< true >( s, until_turn, max_turns, status );
 }
 
 // Occupancy target of the two-particles-per-lane track kernel: its two
@@ -193,12 +213,15 @@ extern "C" hipError_t KN( st_launch_batch_select )( const st_batch_sched_t* s,
 
 extern "C" hipError_t KN( st_launch_batch_track )( const st_batch_sched_t* s,
     int64_t until_turn, int64_t max_turns, int32_t* status, int block, int grid,
-    int particles_per_lane, hipStream_t stream )
+    int particles_per_lane, int heavy, hipStream_t stream )
 {
     if( s->n_seeds <= 0 || s->n_queues <= 0 ) return hipSuccess;
     if( block <= 0 || block > 256 || ( block % 64 ) != 0 ) block = 256;
     if( grid <= 0 ) grid = 1;
-    if( particles_per_lane == 2 )
+    if( heavy )
+        hipLaunchKernelGGL( KN( st_k_batch_track_heavy ), dim3( ( unsigned )grid ),
+                            dim3( ( unsigned )block ), 0, stream, *s, until_turn, max_turns, status );
+    else if( particles_per_lane == 2 )
         hipLaunchKernelGGL( KN( st_k_batch_track2 ), dim3( ( unsigned )grid ),
                             dim3( ( unsigned )block ), 0, stream, *s, until_turn, max_turns, status );
     else

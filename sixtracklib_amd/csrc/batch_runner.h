@@ -52,13 +52,16 @@ struct Config
     int64_t turns_per_launch = 0;
     bool graph = true;
     int ppl = 1;                 // particles per lane (1 or 2)
+    bool heavy = false;          // lattices hold maps outside the lean set
+                                 // (set by the owner, one particle per lane)
 
     std::string str() const
     {
         char b[ 192 ];
         std::snprintf( b, sizeof( b ), "fma=%d block=%d waves_per_cu=%d queues=%d "
-                       "turns_per_launch=%lld graph=%d ppl=%d", fma ? 1 : 0, block, waves_per_cu,
-                       queues, ( long long )turns_per_launch, graph ? 1 : 0, ppl );
+                       "turns_per_launch=%lld graph=%d ppl=%d%s", fma ? 1 : 0, block, waves_per_cu,
+                       queues, ( long long )turns_per_launch, graph ? 1 : 0, ppl,
+                       heavy ? " heavy" : "" );
         return b;
     }
 
@@ -276,9 +279,9 @@ struct BatchRunner
             if( !enqueue_reset_select( until_turn, stream ) ) return false;
             hipError_t const e = cfg.fma
                 ? st_launch_batch_track_fma( &s, until_turn, k, d_status, cfg.block, grid(), cfg.ppl,
-                                             stream )
+                                             cfg.heavy ? 1 : 0, stream )
                 : st_launch_batch_track_exact( &s, until_turn, k, d_status, cfg.block, grid(), cfg.ppl,
-                                               stream );
+                                               cfg.heavy ? 1 : 0, stream );
             if( !hip_ok( e, "launch batch track" ) ) return false;
         }
         return true;

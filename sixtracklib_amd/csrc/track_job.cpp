@@ -468,14 +468,15 @@ struct NS(TrackJobBase)
         return last_status;
     }
 
-    // Lean lattices (the default): the persistent scheduling of the batched
+    // track_until (the default): the persistent scheduling of the batched
     // form with the job's particle sets as seeds (batch_runner.h).  Only
     // the particles with turns to go take lanes; a long request is split
     // into chunks of a few turns (list rebuilt between them, on the
     // device), so lanes of particles lost during a chunk are reused at the
     // next.  Every particle is tracked turn by turn on its own, so the
-    // results are those of one launch.  compact=0 (or a full lattice) runs
-    // one launch with one lane per particle.
+    // results are those of one launch.  Lattices with maps outside the
+    // lean set run the heavy variant of the same loop (those records out
+    // of line).  compact=0 runs one launch with one lane per particle.
     bool ensure_runner()
     {
         if( runner_ready ) return true;
@@ -484,10 +485,11 @@ struct NS(TrackJobBase)
         runner.cfg.block = ( block == 64 || block == 128 ) ? block : 256;
         runner.cfg.turns_per_launch = turns_per_launch;
         runner.cfg.queues = queues;
-        runner.cfg.ppl = ppl;
+        runner.cfg.heavy = lat_full;
+        runner.cfg.ppl = lat_full ? 1 : ppl;
         // resident waves per CU: 8 per SIMD with one particle per lane, 5
         // with two (the kernels' register budgets)
-        runner.cfg.waves_per_cu = ppl == 2 ? 20 : 32;
+        runner.cfg.waves_per_cu = runner.cfg.ppl == 2 ? 20 : 32;
         runner.cfg.graph = false;
         std::vector< st_batch_seed_t > seeds;
         for( const st_dev_particles_t& v : views )
@@ -504,7 +506,7 @@ struct NS(TrackJobBase)
     {
         if( !begin_status() ) return end_status( false );
         bool ok = true;
-        if( compact && !lat_full && n_elem > 0 )
+        if( compact && n_elem > 0 )
         {
             int64_t n_active = 0, min_turn = 0;
             ok = ensure_runner() && runner.select_sync( until_turn, stream, n_active, min_turn );

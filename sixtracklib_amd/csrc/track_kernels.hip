@@ -65,22 +65,6 @@ ST_DEV void store_lane( const st_dev_particles_t& P, int64_t i, const Lane& L,
     P.at_turn[ i ] = at_turn;
 }
 
-// Full 21-field record (particles.h:1695-1803 field order) into output
-// set O at index k.  Read-only fields come from the input set.
-ST_DEV void store_record( const st_dev_particles_t& O, int64_t k,
-    const st_dev_particles_t& P, int64_t i, const Lane& L, const LaneConst& C,
-    int64_t pid, int64_t at_element, int64_t at_turn )
-{
-    O.q0[ k ] = C.q0; O.mass0[ k ] = C.mass0; O.beta0[ k ] = C.beta0;
-    O.gamma0[ k ] = P.gamma0[ i ]; O.p0c[ k ] = C.p0c; O.s[ k ] = L.s;
-    O.x[ k ] = L.x; O.y[ k ] = L.y; O.px[ k ] = L.px; O.py[ k ] = L.py;
-    O.zeta[ k ] = L.zeta; O.psigma[ k ] = L.psigma; O.delta[ k ] = L.delta;
-    O.rpp[ k ] = L.rpp; O.rvv[ k ] = L.rvv; O.chi[ k ] = C.chi;
-    O.charge_ratio[ k ] = C.charge_ratio; O.particle_id[ k ] = pid;
-    O.at_element[ k ] = at_element; O.at_turn[ k ] = at_turn;
-    O.state[ k ] = L.state;
-}
-
 // store_record for record k < 2^29 of every field array: one 32-bit byte
 // offset serves all 21 stores (scalar base + vector offset addressing)
 ST_DEV void store_record_narrow( const st_dev_particles_t& O, int64_t k, double gamma0,
@@ -99,50 +83,6 @@ ST_DEV void store_record_narrow( const st_dev_particles_t& O, int64_t k, double 
     d( O.charge_ratio ) = C.charge_ratio; n( O.particle_id ) = pid;
     n( O.at_element ) = at_element; n( O.at_turn ) = at_turn;
     n( O.state ) = L.state;
-}
-
-// device copy of an output particle set header, addressed by a monitor
-ST_DEV st_dev_particles_t load_set( uint64_t addr )
-{
-    cptr_t w = ( cptr_t )addr;
-    st_dev_particles_t O;
-    O.n = ( int64_t )w[ 0 ];
-    double** dp = &O.q0;
-    for( int k = 0; k < 17; ++k ) dp[ k ] = ( double* )w[ 1 + k ];
-    int64_t** ip = &O.particle_id;
-    for( int k = 0; k < 4; ++k ) ip[ k ] = ( int64_t* )w[ 18 + k ];
-    return O;
-}
-
-// be_monitor/track.h:47-170.  Returns true when the copy fails (index past
-// the output set): the reference then marks the particle lost, status -1.
-ST_DEV_NOINLINE bool monitor( cptr_t e, const st_dev_particles_t& P, int64_t i,
-    const Lane& L, const LaneConst& C, int64_t pid, int64_t at_element,
-    int64_t at_turn, bool active )
-{
-    uint64_t const out = e[ 3 ];
-    if( out == 0u ) return false;
-    int64_t const num_stores = ri( e, 0 ), start = ri( e, 1 ), skip = ri( e, 2 );
-    int64_t const max_pid = ri( e, 4 ), min_pid = ri( e, 5 );
-    int64_t const is_rolling = ri( e, 6 ), is_turn_ordered = ri( e, 7 );
-    int64_t out_idx = -1;
-    int64_t const since = at_turn - start;
-    if( since >= 0 && ( since % skip ) == 0 )
-    {
-        int64_t store_idx = since / skip;
-        if( store_idx >= num_stores && is_rolling ) store_idx = store_idx % num_stores;
-        if( store_idx < num_stores )
-        {
-            int64_t const off = pid - min_pid;
-            int64_t const nstore = max_pid - min_pid + 1;
-            out_idx = is_turn_ordered ? nstore * store_idx + off : off * nstore + store_idx;
-        }
-    }
-    if( out_idx < 0 ) return false;
-    st_dev_particles_t const O = load_set( out );
-    if( out_idx >= O.n ) return true;
-    if( active ) store_record( O, out_idx, P, i, L, C, pid, at_element, at_turn );
-    return false;
 }
 
 // Elements other than Drift and Multipole (track/track.h:209-585).

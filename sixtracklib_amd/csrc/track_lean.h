@@ -22,6 +22,7 @@
 #pragma once
 
 #include "track_device.h"
+#include "track_heavy.h"
 
 namespace st
 {
@@ -42,11 +43,15 @@ ST_DEV void store_lane_state( const st_dev_particles_t& P, int64_t i, const Lane
     P.at_turn[ j ] = at_turn;
 }
 
-// Records other than Drift / Multipole in a lean lattice.  Returns the
-// survival predicate; an op outside the lean set is a map failure
-// (track.h:577-581: lost, status -1).
-ST_DEV bool lean_other( unsigned op, cptr_t e, Lane& L, const st_dev_particles_t& P,
-                        int64_t li, bool& fail )
+// Records other than Drift / Multipole.  Returns the survival predicate.
+// Without HEAVY an op outside the lean set is a map failure (track.h:
+// 577-581: lost, status -1); with HEAVY the other maps run out of line
+// (heavy_step) on a copy of the lane, so the loop's own state stays in
+// registers.
+template< bool HEAVY >
+ST_DEV bool lean_other( unsigned op, unsigned aux, cptr_t e, Lane& L,
+                        const st_dev_particles_t& P, int64_t li, bool active,
+                        int64_t at_element, int64_t at_turn, bool& fail )
 {
     switch( op )
     {
@@ -67,8 +72,17 @@ ST_DEV bool lean_other( unsigned op, cptr_t e, Lane& L, const st_dev_particles_t
     case ST_OP_LIMIT_ELLIPSE: return limit_ellipse_ok( L, e );
     case ST_OP_LIMIT_RECT_ELLIPSE: return limit_rect_ellipse_ok( L, e );
     case ST_OP_DIPEDGE: dipedge( L, e ); return true;
-    default: fail = true; return false;
+    default: break;
     }
+    if( HEAVY )
+    {
+        Lane io = L;
+        bool const ok = heavy_step( op, aux, e, &io, &P, li, active, at_element, at_turn, &fail );
+        L = io;
+        return ok;
+    }
+    fail = true;
+    return false;
 }
 
 // Record at byte offset ``off`` of the stream (32-bit offsets: one scalar
@@ -78,12 +92,14 @@ ST_DEV cptr_t rec_at( cptr_t lat, uint32_t off )
     return ( cptr_t )( ( const __attribute__( ( address_space( 4 ) ) ) char* )lat + off );
 }
 
-// One lane of a lean until-turn launch.
+// One lane of a lean until-turn launch (HEAVY: the lattice may also hold
+// the other maps, see lean_other).
 //   P          particle set; particle i = list[tid] with a list, else tid
 //   n_lanes    number of valid tid (list length, or P.n without a list)
 //   lat        packed lean lattice of n_elem records (END-terminated)
 //   max_turns  turns this launch may advance a lane (<= 0: until_turn)
 // The OR of the per-particle statuses goes to *status_out.
+template< bool HEAVY = false >
 ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, int64_t n_lanes,
                              int64_t tid, cptr_t lat, int64_t n_elem, int64_t until_turn,
                              int64_t max_turns, int32_t* status_out )
@@ -150,7 +166,9 @@ ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, i
             uint32_t const noff = off + ST_REC_BYTES( h );
             uint64_t const hn = rec_at( lat, noff )[ 0 ];
             bool fail = false;
-            bool const ok = lean_other( op, r + ST_REC_PAYLOAD, L, P, li, fail );
+            bool const ok = lean_other< HEAVY >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L, P, li,
+                                                 lane_in( act ), start + ri( r, 1 ), at_turn,
+                                                 fail );
             uint64_t const lost = ballot( !ok ) & act;
             if( lost != 0u )
             {
@@ -275,7 +293,9 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
             {
                 int64_t const li = lane_in( act[ k ] ) ? i[ k ] : 0;
                 bool fail = false;
-                bool const ok = lean_other( op, r + ST_REC_PAYLOAD, L[ k ], P, li, fail );
+                bool const ok = lean_other< false >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L[ k ],
+                                                     P, li, lane_in( act[ k ] ),
+                                                     start[ k ] + ri( r, 1 ), at_turn[ k ], fail );
                 uint64_t const lost = ballot( !ok ) & act[ k ];
                 if( lost != 0u )
                 {

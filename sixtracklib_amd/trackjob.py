@@ -195,6 +195,13 @@ class TrackJob:
     def uses_fma(self):
         return bool(st.st_TrackJob_uses_fma(self.ptr_st_track_job))
 
+    @property
+    def uses_full_kernel(self):
+        """True when the lattice holds maps outside the lean set (monitors,
+        beam-beam, space charge, RF multipoles), which the full kernel
+        tracks."""
+        return bool(st.st_TrackJob_uses_full_kernel(self.ptr_st_track_job))
+
     # -- tracking ----------------------------------------------------------
     def _check(self, status, what):
         self._last_track_status = status
