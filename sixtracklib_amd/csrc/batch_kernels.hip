@@ -127,10 +127,22 @@ ST_DEV void batch_track_tiles( const st_batch_sched_t& s, int64_t until_turn, in
                 t -= nt;
             }
             if( seed < 0 ) break;                    // this queue is drained
-            SeedView const S = load_seed( s.seeds, seed );
-            track_lean_lane< HEAVY >( S.P, S.list, counts[ 2 * seed ],
-                                      ( int64_t )t * 64 + __lane_id(), S.lat, S.n_elem,
-                                      until_turn, max_turns, status );
+            if constexpr( HEAVY )
+            {
+                // the particle set is read from the seed descriptor where
+                // it is used (its first 22 words are the set's view)
+                cptr_t const w = ( cptr_t )s.seeds + 25 * seed;
+                track_lean_lane< true >( PDesc{ w }, ( const int32_t* )w[ 24 ], counts[ 2 * seed ],
+                                         ( int64_t )t * 64 + __lane_id(), ( cptr_t )w[ 22 ],
+                                         ( int64_t )w[ 23 ], until_turn, max_turns, status );
+            }
+            else
+            {
+                SeedView const S = load_seed( s.seeds, seed );
+                track_lean_lane< false >( PDirect{ S.P }, S.list, counts[ 2 * seed ],
+                                          ( int64_t )t * 64 + __lane_id(), S.lat, S.n_elem,
+                                          until_turn, max_turns, status );
+            }
         }
     }
 }

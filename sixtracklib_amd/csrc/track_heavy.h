@@ -77,9 +77,11 @@ ST_DEV_NOINLINE bool monitor( cptr_t e, const st_dev_particles_t& P, int64_t i,
 // reports failure (a monitor whose index lies past its output set, a
 // disabled map, an unknown type).
 ST_DEV_NOINLINE bool heavy_step( unsigned op, unsigned aux, cptr_t e, Lane* io,
-    const st_dev_particles_t* P, int64_t li, bool active, int64_t at_element,
+    cptr_t set_desc, int64_t li, bool active, int64_t at_element,
     int64_t at_turn, bool* fail )
 {
+    st_dev_particles_t const PV = load_set( ( uint64_t )set_desc );
+    st_dev_particles_t const* const P = &PV;
     int64_t j = li;
     asm volatile( "" : "+v"( j ) );
     LaneConst C;

@@ -290,7 +290,7 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
 
 ST_DEV void track_until_lean( const st_dev_particles_t& P, int64_t tid, const Args& a )
 {
-    track_lean_lane( P, nullptr, P.n, tid, a.lat, a.n_elem, a.until_turn, 0, a.status_out );
+    track_lean_lane( PDirect{ P }, nullptr, P.n, tid, a.lat, a.n_elem, a.until_turn, 0, a.status_out );
 }
 
 }  // namespace

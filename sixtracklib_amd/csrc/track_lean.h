@@ -43,14 +43,32 @@ ST_DEV void store_lane_state( const st_dev_particles_t& P, int64_t i, const Lane
     P.at_turn[ j ] = at_turn;
 }
 
+// Access to the particle set of a lean loop.  PDirect holds the view
+// (kernel arguments); PDesc reads it from a descriptor in constant memory
+// at each use (count and 21 field addresses, the st_dev_particles_t
+// layout), so that none of the field addresses stays live across the
+// element loop: the heavy loop's out-of-line calls would otherwise force
+// them to be saved and restored around every element.
+struct PDirect
+{
+    const st_dev_particles_t& P;
+    ST_DEV const st_dev_particles_t& view() const { return P; }
+};
+
+struct PDesc
+{
+    cptr_t d;
+    ST_DEV st_dev_particles_t view() const { return load_set( ( uint64_t )d ); }
+};
+
 // Records other than Drift / Multipole.  Returns the survival predicate.
 // Without HEAVY an op outside the lean set is a map failure (track.h:
 // 577-581: lost, status -1); with HEAVY the other maps run out of line
 // (heavy_step) on a copy of the lane, so the loop's own state stays in
 // registers.
-template< bool HEAVY >
+template< bool HEAVY, class PA >
 ST_DEV bool lean_other( unsigned op, unsigned aux, cptr_t e, Lane& L,
-                        const st_dev_particles_t& P, int64_t li, bool active,
+                        const PA& pa, int64_t li, bool active,
                         int64_t at_element, int64_t at_turn, bool& fail )
 {
     switch( op )
@@ -60,6 +78,7 @@ ST_DEV bool lean_other( unsigned op, unsigned aux, cptr_t e, Lane& L,
     {
         int64_t j = li;
         asm volatile( "" : "+v"( j ) );
+        auto const& P = pa.view();
         LaneConst C;
         C.q0 = P.q0[ j ]; C.mass0 = P.mass0[ j ]; C.beta0 = P.beta0[ j ];
         C.p0c = P.p0c[ j ]; C.chi = P.chi[ j ]; C.charge_ratio = P.charge_ratio[ j ];
@@ -74,10 +93,10 @@ ST_DEV bool lean_other( unsigned op, unsigned aux, cptr_t e, Lane& L,
     case ST_OP_DIPEDGE: dipedge( L, e ); return true;
     default: break;
     }
-    if( HEAVY )
+    if constexpr( HEAVY )
     {
         Lane io = L;
-        bool const ok = heavy_step( op, aux, e, &io, &P, li, active, at_element, at_turn, &fail );
+        bool const ok = heavy_step( op, aux, e, &io, pa.d, li, active, at_element, at_turn, &fail );
         L = io;
         return ok;
     }
@@ -99,8 +118,8 @@ ST_DEV cptr_t rec_at( cptr_t lat, uint32_t off )
 //   lat        packed lean lattice of n_elem records (END-terminated)
 //   max_turns  turns this launch may advance a lane (<= 0: until_turn)
 // The OR of the per-particle statuses goes to *status_out.
-template< bool HEAVY = false >
-ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, int64_t n_lanes,
+template< bool HEAVY = false, class PA = PDirect >
+ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
                              int64_t tid, cptr_t lat, int64_t n_elem, int64_t until_turn,
                              int64_t max_turns, int32_t* status_out )
 {
@@ -108,13 +127,19 @@ ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, i
     int64_t const i = ( list != nullptr && in_range ) ? ( int64_t )list[ tid ] : tid;
     int64_t const li = in_range ? i : 0;          // out-of-range lanes never store
     Lane L;
-    L.x = P.x[ li ]; L.px = P.px[ li ]; L.y = P.y[ li ]; L.py = P.py[ li ];
-    L.zeta = P.zeta[ li ]; L.delta = P.delta[ li ]; L.rpp = P.rpp[ li ];
-    L.rvv = P.rvv[ li ]; L.psigma = P.psigma[ li ]; L.s = P.s[ li ];
-    double const chi = P.chi[ li ];
-    int64_t const start = P.at_element[ li ];
-    int64_t at_turn = P.at_turn[ li ];
-    bool const active0 = in_range && n_elem > 0 && until_turn > at_turn && P.state[ li ] == 1;
+    double chi;
+    int64_t start, at_turn;
+    bool active0;
+    {
+        auto const& P = pa.view();
+        L.x = P.x[ li ]; L.px = P.px[ li ]; L.y = P.y[ li ]; L.py = P.py[ li ];
+        L.zeta = P.zeta[ li ]; L.delta = P.delta[ li ]; L.rpp = P.rpp[ li ];
+        L.rvv = P.rvv[ li ]; L.psigma = P.psigma[ li ]; L.s = P.s[ li ];
+        chi = P.chi[ li ];
+        start = P.at_element[ li ];
+        at_turn = P.at_turn[ li ];
+        active0 = in_range && n_elem > 0 && until_turn > at_turn && P.state[ li ] == 1;
+    }
     int32_t status = ( in_range && n_elem <= 0 ) ? -1 : 0;
     // a launch may cover only part of the requested turns; the lane then
     // stops at a turn boundary exactly as if until_turn were turn_stop
@@ -153,7 +178,7 @@ ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, i
                     if( lost != 0u )
                     {
                         if( lane_in( lost ) )
-                            store_lane_state( P, i, L, 0, start + ri( r, 1 ), at_turn );
+                            store_lane_state( pa.view(), i, L, 0, start + ri( r, 1 ), at_turn );
                         act &= ~lost;
                     }
                 }
@@ -166,7 +191,7 @@ ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, i
             uint32_t const noff = off + ST_REC_BYTES( h );
             uint64_t const hn = rec_at( lat, noff )[ 0 ];
             bool fail = false;
-            bool const ok = lean_other< HEAVY >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L, P, li,
+            bool const ok = lean_other< HEAVY >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L, pa, li,
                                                  lane_in( act ), start + ri( r, 1 ), at_turn,
                                                  fail );
             uint64_t const lost = ballot( !ok ) & act;
@@ -174,7 +199,7 @@ ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, i
             {
                 if( lane_in( lost ) )
                 {
-                    store_lane_state( P, i, L, 0, start + ri( r, 1 ), at_turn );
+                    store_lane_state( pa.view(), i, L, 0, start + ri( r, 1 ), at_turn );
                     if( fail ) status = -1;
                 }
                 act &= ~lost;
@@ -186,7 +211,7 @@ ST_DEV void track_lean_lane( const st_dev_particles_t& P, const int32_t* list, i
         uint64_t const fin = ballot( at_turn >= turn_stop ) & act;
         if( fin != 0u )
         {
-            if( lane_in( fin ) ) store_lane_state( P, i, L, 1, start, at_turn );
+            if( lane_in( fin ) ) store_lane_state( pa.view(), i, L, 1, start, at_turn );
             act &= ~fin;
         }
     }
@@ -294,7 +319,7 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
                 int64_t const li = lane_in( act[ k ] ) ? i[ k ] : 0;
                 bool fail = false;
                 bool const ok = lean_other< false >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L[ k ],
-                                                     P, li, lane_in( act[ k ] ),
+                                                     PDirect{ P }, li, lane_in( act[ k ] ),
                                                      start[ k ] + ri( r, 1 ), at_turn[ k ], fail );
                 uint64_t const lost = ballot( !ok ) & act[ k ];
                 if( lost != 0u )
