@@ -43,9 +43,9 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
 # the 23-turn track launch of the 1M-particle ring (578,319 / 128,460 KiB)
 # and the 1-turn element-by-element launch of the collective ring with
-# 100k particles (1,789,563 / 342,027,996 KiB)
+# 100k particles (740,191 / 211,709,952 KiB)
 TRACK_LAUNCH_HBM_BYTES = 578319 * 2048 + 128460 * 1024
-EBE_TURN_HBM_BYTES = 1789563 * 2048 + 342027996 * 1024
+EBE_TURN_HBM_BYTES = 740191 * 2048 + 211709952 * 1024
 
 
 def sigma_at_start(eb):
@@ -232,26 +232,34 @@ def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
     same_as_ebe = bool(np.array_equal(pb.words, ebe_after))
     pb.words[:] = ebe_after
     # a record is stored before every element a particle enters, the one
-    # where it is lost included
-    records = int(np.sum(np.where(alive, at_turn * n_elem, at_turn * n_elem + at_elem + 1)))
+    # where it is lost included; the first ``passes`` records of a particle
+    # in a call fill its slots (168 B each), later ones overwrite the 13
+    # fields that change (104 B: the 8 fixed fields already hold the
+    # particle's values)
+    per_particle = np.where(alive, at_turn * n_elem, at_turn * n_elem + at_elem + 1)
+    records = int(np.sum(per_particle))
+    first = np.minimum(per_particle, passes)
+    stored_bytes = int(np.sum(first * 168 + (per_particle - first) * 104))
     wall, kms = best
     del job
-    gbs = records * 168 / (kms / 1e3) / 1e9
+    gbs = stored_bytes / (kms / 1e3) / 1e9
     return {"workload": "track_elem_by_elem, collective_ring(): %d elements (20k ring + 2 "
                         "BeamBeam6D x 5 slices + BeamBeam4D + 16 SCQGaussProfile); %d "
                         "particles, matched 6D Gaussian; %d turns; rolling store of %d "
                         "passes per particle" % (n_elem, n_particles, turns, passes),
             "particles": n_particles, "turns": turns, "elements": n_elem,
             "store_passes": passes, "store_gb": n_particles * passes * 168 / 1e9,
-            "records": records, "alive_at_end": int(np.count_nonzero(alive)),
+            "records": records, "records_per_second": records / (kms / 1e3),
+            "stored_bytes": stored_bytes, "alive_at_end": int(np.count_nonzero(alive)),
             "kernel_seconds": kms / 1e3, "wall_seconds": wall,
             "value": int(np.sum(at_turn)) / wall, "unit": "particle-turns/s",
             "track_until_value": until_tracked / until_wall,
             "track_until_same_particles_as_elem_by_elem": same_as_ebe,
             "roofline": {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                         "frac": gbs / HBM_PEAK_GBS, "bytes_per_record": 168,
-                         "traffic_per_turn": EBE_TURN_HBM_BYTES,
-                         "algorithmic_bytes_per_turn": n_particles * n_elem * 168}}
+                         "frac": gbs / HBM_PEAK_GBS,
+                         "bytes_per_record": "168 for a particle's first %d records of the "
+                                             "call, 104 after" % passes,
+                         "traffic_per_turn": EBE_TURN_HBM_BYTES}}
 
 
 def calibration_copy(nbytes):

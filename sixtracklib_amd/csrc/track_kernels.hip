@@ -66,21 +66,29 @@ ST_DEV void store_lane( const st_dev_particles_t& P, int64_t i, const Lane& L,
 }
 
 // store_record for record k < 2^29 of every field array: one 32-bit byte
-// offset serves all 21 stores (scalar base + vector offset addressing)
+// offset serves all 21 stores (scalar base + vector offset addressing).
+// Without ``constants`` the eight fields that are fixed for a particle
+// (q0, mass0, beta0, gamma0, p0c, chi, charge_ratio, particle_id) are not
+// written: the caller knows the slot already holds this particle's values.
 ST_DEV void store_record_narrow( const st_dev_particles_t& O, int64_t k, double gamma0,
-    const Lane& L, const LaneConst& C, int64_t pid, int64_t at_element, int64_t at_turn )
+    const Lane& L, const LaneConst& C, int64_t pid, int64_t at_element, int64_t at_turn,
+    bool constants )
 {
     uint32_t const b = static_cast< uint32_t >( k ) * 8u;
     auto d = [ b ]( double* base ) -> double&
     { return *reinterpret_cast< double* >( reinterpret_cast< char* >( base ) + b ); };
     auto n = [ b ]( int64_t* base ) -> int64_t&
     { return *reinterpret_cast< int64_t* >( reinterpret_cast< char* >( base ) + b ); };
-    d( O.q0 ) = C.q0; d( O.mass0 ) = C.mass0; d( O.beta0 ) = C.beta0;
-    d( O.gamma0 ) = gamma0; d( O.p0c ) = C.p0c; d( O.s ) = L.s;
+    if( constants )
+    {
+        d( O.q0 ) = C.q0; d( O.mass0 ) = C.mass0; d( O.beta0 ) = C.beta0;
+        d( O.gamma0 ) = gamma0; d( O.p0c ) = C.p0c; d( O.chi ) = C.chi;
+        d( O.charge_ratio ) = C.charge_ratio; n( O.particle_id ) = pid;
+    }
+    d( O.s ) = L.s;
     d( O.x ) = L.x; d( O.y ) = L.y; d( O.px ) = L.px; d( O.py ) = L.py;
     d( O.zeta ) = L.zeta; d( O.psigma ) = L.psigma; d( O.delta ) = L.delta;
-    d( O.rpp ) = L.rpp; d( O.rvv ) = L.rvv; d( O.chi ) = C.chi;
-    d( O.charge_ratio ) = C.charge_ratio; n( O.particle_id ) = pid;
+    d( O.rpp ) = L.rpp; d( O.rvv ) = L.rvv;
     n( O.at_element ) = at_element; n( O.at_turn ) = at_turn;
     n( O.state ) = L.state;
 }
@@ -158,6 +166,7 @@ struct Args
     bool finish_turn;        // line mode
     st_ebe_config_t cfg;     // ebe mode
     int64_t ebe_cap;         // ebe store capacity in records
+    int64_t ebe_slots;       // store slots per particle (capacity / particles)
     st_dev_particles_t O;    // ebe output set
     int32_t* status_out;
 };
@@ -198,6 +207,15 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
     bool const narrow = MODE == MODE_EBE && a.O.n <= ( int64_t( 1 ) << 29 );
     bool const pid_ok = MODE == MODE_EBE && a.cfg.min_particle_id <= pid &&
                         a.cfg.max_particle_id >= pid;
+    // A rolling store whose configuration covers every pass of this lane
+    // (its particle id, all its elements, its turns from the first on)
+    // gives the lane one fixed column of slots, visited in order: after
+    // the first ebe_slots passes of the call every slot holds this
+    // particle's fixed fields, and later records skip them.
+    bool const every_pass = pid_ok && a.cfg.is_rolling && a.cfg.min_turn <= at_turn &&
+                            a.cfg.min_element_id <= start &&
+                            a.cfg.max_element_id >= start + n_elem - 1;
+    int64_t pass = 0;                             // element passes of this call (wave)
 
     // lanes still being tracked; a lane leaves the mask exactly once, after
     // its final state has been stored
@@ -232,9 +250,12 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
                 }
                 if( lane_in( act ) && k >= 0 && k < a.O.n )
                 {
-                    if( narrow ) store_record_narrow( a.O, k, gamma0, L, C, pid, el, at_turn );
+                    if( narrow )
+                        store_record_narrow( a.O, k, gamma0, L, C, pid, el, at_turn,
+                                             !every_pass || pass < a.ebe_slots );
                     else store_record( a.O, k, P, i, L, C, pid, el, at_turn );
                 }
+                ++pass;
             }
             uint64_t bad = 0u, failm = 0u;        // lost here / map failed
             if( op == ST_OP_DRIFT )
@@ -366,6 +387,7 @@ extern "C" hipError_t KN( st_launch_track_ebe )( const st_dev_particles_t* P,
     Args a{};
     a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
     a.cfg = *cfg; a.ebe_cap = cap; a.O = *O; a.status_out = status;
+    a.ebe_slots = cfg->num_particles_to_store > 0 ? cap / cfg->num_particles_to_store : 0;
     hipLaunchKernelGGL( KN( st_k_track_ebe ), dim3( grid_for( P->n, block ) ), dim3( block ),
                         0, stream, *P, a );
     return hipGetLastError();

@@ -289,7 +289,8 @@ def test_elem_by_elem_output_matches_oracle():
 
 def _collective_beam(n=192):
     """Matched beam for the short collective ring; the first 12 particles
-    sit at growing amplitudes so that some are lost in turns 0 and 1."""
+    sit at growing amplitudes so that some are lost in turns 0 and 1
+    (n >= 12)."""
     _, p = lattices.matched_gaussian_beam(n, seed=3)
     x = np.array(p.x)
     x[:12] = np.linspace(0.004, 0.0098, 12)
@@ -297,6 +298,36 @@ def _collective_beam(n=192):
                                     py=np.array(p.py), zeta=np.array(p.zeta),
                                     delta=np.array(p.delta))
     return pb
+
+
+def test_elem_by_elem_reference_store_rolls_over_turns():
+    """The reference's own store (particles x elements x 2 turns, rolling)
+    for 5 turns of element-by-element tracking: turns 2-4 overwrite the
+    slots of turns 0-1, exactly as the oracle does with the same
+    configuration."""
+    from sixtracklib_amd.trackjob import TrackJob
+    from sixtracklib_amd.particles import particles_in, Particles
+    eb = lattices.collective_ring(n_cells=8, n_sc=4)
+    E = eb.n_objects
+    pb = _collective_beam(130)
+    P = particles_in(pb).num_particles
+    ob = _copy(pb)
+    job = TrackJob(eb, pb, until_turn_elem_by_elem=2, config_str="fma=0")
+    off = job.elem_by_elem_output_offset
+    job.track_elem_by_elem(5)
+    job.collect()
+    got = _pdict(job.output, off)
+    assert len(got["x"]) == P * E * 2
+    ocb = CBuffer(max_slots=22 * P * E * 2 + 64)
+    Particles(cbuffer=ocb, num_particles=P * E * 2)
+    cfg = oracle_lib.EbeConfig(order=0, num_particles_to_store=P, num_elements_to_store=E,
+                               num_turns_to_store=2, min_particle_id=0, min_element_id=0,
+                               min_turn=0, max_particle_id=P - 1, max_element_id=E - 1,
+                               max_turn=1, is_rolling=1, out_store_addr=ocb.get_object_address(0))
+    assert oracle_lib.track_elem_by_elem(ob, eb, cfg, 5) == 0
+    assert_particles_equal(_pdict(pb), _pdict(ob), what="particles")
+    assert_particles_equal(got, _pdict(ocb), what="rolling reference-size store")
+    assert np.any(got["at_turn"] >= 3)
 
 
 @pytest.mark.parametrize("passes", [300, 407 * 4 - 5])
