@@ -25,6 +25,8 @@
  *   ppl=1|2               particles per lane (1): with 2 every lane tracks
  *                         two particles through the shared record walk
  *                         (128-particle tiles; results identical)
+ *   wave_prio=0|1         waves with more turns left in their tile get
+ *                         issue priority (1; results identical)
  * NS(TrackBatch_autotune) picks particles per lane, block, waves_per_cu
  * and queues by timing short trials on the batch itself (state restored
  * afterwards).

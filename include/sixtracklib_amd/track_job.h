@@ -28,7 +28,14 @@
  * list is rebuilt between them; results identical), "queues=1|8" (default
  * 1: one work queue for the persistent launch; 8: one per XCD), "ppl=1|2"
  * (default 1: particles per lane of the until-turn kernel of lean
- * lattices; results identical),
+ * lattices; results identical), "balance=0|1" (default 1: with compact=1
+ * every launch gives each resident wave one 64-particle tile, the
+ * particles furthest behind first, and the host reads the counts back
+ * between launches; 0: every chunk hands out all its tiles; results
+ * identical), "wave_prio=0|1" (default 1: the waves of a launch are given
+ * issue priority by their progress, so that they finish together; results
+ * identical), "waves_per_cu=N" (default: the kernel's full occupancy;
+ * fewer resident waves of the persistent launch, for tests),
  * "ebe_store_passes=K" (default 0: the element-by-element store holds
  * particles x elements x turns records, as the reference's; K > 0 sizes it
  * for K element passes of every particle when that is fewer, and with

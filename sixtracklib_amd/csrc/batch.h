@@ -26,8 +26,17 @@ typedef struct st_batch_sched_s
     const int32_t* queue_seeds;     // seeds of queue q: [queue_begin[q], queue_begin[q+1])
     const int32_t* queue_begin;     // [n_queues + 1]
     int32_t* queue_next;            // [n_queues] next tile to hand out
+    // Balanced form (split != nullptr): each seed's list holds the
+    // particles at the smallest at_turn of all seeds (*gmin) at its front,
+    // split[2k] of them, and the others at its back, split[2k + 1] of them
+    // ending at index P.n - 1; a wave tracks at most one tile per launch,
+    // front tiles of every seed before back tiles.
+    int32_t* split;                 // [2 * n_seeds] or nullptr
+    int32_t* gmin;                  // [1] smallest at_turn with turns to go
+    unsigned long long* progress;   // [1] turns done by the waves of a balanced launch
     int32_t n_seeds;
     int32_t n_queues;
+    int32_t wave_prio;              // waves prioritised by turns left (track_lean.h)
 }
 st_batch_sched_t;
 
@@ -35,6 +44,8 @@ st_batch_sched_t;
     extern "C" hipError_t st_launch_batch_reset##SUF( const st_batch_sched_t* s,     \
         hipStream_t stream );                                                         \
     extern "C" hipError_t st_launch_batch_select##SUF( const st_batch_sched_t* s,    \
+        int64_t max_particles, int64_t until_turn, hipStream_t stream );              \
+    extern "C" hipError_t st_launch_batch_select_split##SUF( const st_batch_sched_t* s, \
         int64_t max_particles, int64_t until_turn, hipStream_t stream );              \
     extern "C" hipError_t st_launch_batch_track##SUF( const st_batch_sched_t* s,     \
         int64_t until_turn, int64_t max_turns, int32_t* status, int block, int grid, \

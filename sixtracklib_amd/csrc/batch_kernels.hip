@@ -86,7 +86,10 @@ __global__ void __launch_bounds__( 256 ) KN( st_k_batch_reset )( st_batch_sched_
     {
         s.counts[ 2 * k ] = 0;
         s.counts[ 2 * k + 1 ] = INT32_MAX;
+        if( s.split != nullptr ) { s.split[ 2 * k ] = 0; s.split[ 2 * k + 1 ] = 0; }
     }
+    if( i == 0 && s.gmin != nullptr ) *s.gmin = INT32_MAX;
+    if( i == 0 && s.progress != nullptr ) *s.progress = 0ull;
 }
 
 __global__ void __launch_bounds__( 256 ) KN( st_k_batch_select )( st_batch_sched_t s,
@@ -98,17 +101,143 @@ __global__ void __launch_bounds__( 256 ) KN( st_k_batch_select )( st_batch_sched
     select_active_lane( S.P, i, until_turn, S.list, &s.counts[ 2 * blockIdx.y ] );
 }
 
+// Balanced form, pass 1: the smallest at_turn (clamped to int32) among the
+// particles of all seeds with turns to go -> *s.gmin.  One atomic per block.
+__global__ void __launch_bounds__( 256 ) KN( st_k_batch_minturn )( st_batch_sched_t s,
+    int64_t until_turn )
+{
+    cptr_t const w = ( cptr_t )s.seeds + 25 * blockIdx.y;
+    int64_t const n = ( int64_t )w[ 0 ];
+    if( ( int64_t )blockIdx.x * blockDim.x >= n ) return;            // whole block past the set
+    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t const t = ( i < n ) ? ( ( const int64_t* )w[ 20 ] )[ i ] : 0;
+    bool const want = i < n && ( ( const int64_t* )w[ 21 ] )[ i ] == 1 && t < until_turn;
+    int32_t m = want ? ( int32_t )( t < 0 ? 0 : ( t > INT32_MAX ? INT32_MAX : t ) ) : INT32_MAX;
+    for( int off = 32; off > 0; off >>= 1 ) m = min( m, __shfl_xor( m, off ) );
+    __shared__ int32_t part[ 4 ];
+    if( __lane_id() == 0 ) part[ threadIdx.x >> 6 ] = m;
+    __syncthreads();
+    if( threadIdx.x == 0 )
+    {
+        int32_t b = part[ 0 ];
+        for( int k = 1; k < ( int )( blockDim.x >> 6 ); ++k ) b = min( b, part[ k ] );
+        if( b < INT32_MAX ) atomicMin( s.gmin, b );
+    }
+}
+
+// Balanced form, pass 2: the particles with turns to go, those at *s.gmin
+// to the front of the seed's list (split[2k]), the others to its back
+// (split[2k + 1], filled from index P.n - 1 down).  One atomic add per
+// block and part; the order within a part is of no consequence.
+__global__ void __launch_bounds__( 256 ) KN( st_k_batch_select_split )( st_batch_sched_t s,
+    int64_t until_turn )
+{
+    int const k = ( int )blockIdx.y;
+    cptr_t const w = ( cptr_t )s.seeds + 25 * k;
+    int64_t const n = ( int64_t )w[ 0 ];
+    if( ( int64_t )blockIdx.x * blockDim.x >= n ) return;            // whole block past the set
+    int32_t* const list = ( int32_t* )w[ 24 ];
+    int64_t const i = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t const t = ( i < n ) ? ( ( const int64_t* )w[ 20 ] )[ i ] : 0;
+    bool const want = i < n && ( ( const int64_t* )w[ 21 ] )[ i ] == 1 && t < until_turn;
+    bool const front = want && t <= ( int64_t )*s.gmin;
+    bool const back = want && !front;
+    uint64_t const mf = ballot( front ), mb = ballot( back );
+    __shared__ int32_t cf[ 4 ], cb[ 4 ], base[ 2 ];
+    int const wv = ( int )( threadIdx.x >> 6 );
+    if( __lane_id() == 0 ) { cf[ wv ] = __popcll( mf ); cb[ wv ] = __popcll( mb ); }
+    __syncthreads();
+    if( threadIdx.x == 0 )
+    {
+        int32_t tf = 0, tb = 0;
+        for( int q = 0; q < ( int )( blockDim.x >> 6 ); ++q ) { tf += cf[ q ]; tb += cb[ q ]; }
+        base[ 0 ] = tf > 0 ? atomicAdd( &s.split[ 2 * k ], tf ) : 0;
+        base[ 1 ] = tb > 0 ? atomicAdd( &s.split[ 2 * k + 1 ], tb ) : 0;
+    }
+    __syncthreads();
+    int32_t of = base[ 0 ], ob = base[ 1 ];
+    for( int q = 0; q < wv; ++q ) { of += cf[ q ]; ob += cb[ q ]; }
+    uint64_t const below = ( __lane_id() == 0 ) ? 0u : ( ~0ull >> ( 64 - __lane_id() ) );
+    if( front ) list[ of + __popcll( mf & below ) ] = ( int32_t )i;
+    if( back ) list[ n - 1 - ( ob + __popcll( mb & below ) ) ] = ( int32_t )i;
+}
+
+// Issue-priority rule of a track launch (WavePrio, track_lean.h): none
+// without wave_prio; balanced launches keep their waves level with the
+// mean progress of the min(tiles, resident waves) waves that have a tile;
+// the persistent-chunk form bands by turns left.
+template< int TILE >
+ST_DEV WavePrio launch_priority( const st_batch_sched_t& s, int64_t max_turns )
+{
+    WavePrio wp;
+    if( !s.wave_prio || max_turns <= 0 ) return wp;
+    wp.k = max_turns;
+    if( s.split == nullptr || s.progress == nullptr ) return wp;
+    ci32_t* const split = ( ci32_t* )s.split;
+    int64_t tiles = 0;
+    for( int k = 0; k < 2 * s.n_seeds; ++k ) tiles += ( split[ k ] + TILE - 1 ) / TILE;
+    int64_t const waves = ( int64_t )gridDim.x * ( blockDim.x >> 6 );
+    wp.progress = s.progress;
+    wp.n_waves = tiles < waves ? tiles : waves;
+    return wp;
+}
+
+// Tile t of a queue (seeds qseeds[qb..qe)) -> its seed, the part of the
+// seed's list it indexes (list, count) and the tile's index in that part.
+// False when the queue holds t or fewer tiles.  TILE particles per tile.
+template< int TILE >
+ST_DEV bool find_tile( const st_batch_sched_t& s, int qb, int qe, int t, int& seed,
+                       const int32_t*& list, int32_t& count, int& tile )
+{
+    ci32_t* const qseeds = ( ci32_t* )s.queue_seeds;
+    if( s.split == nullptr )
+    {
+        ci32_t* const counts = ( ci32_t* )s.counts;
+        for( int k = qb; k < qe; ++k )
+        {
+            int const sd = qseeds[ k ];
+            int const n = counts[ 2 * sd ];
+            int const nt = ( n + TILE - 1 ) / TILE;
+            if( t < nt )
+            {
+                seed = sd; count = n; tile = t;
+                list = ( const int32_t* )( ( cptr_t )s.seeds + 25 * sd )[ 24 ];
+                return true;
+            }
+            t -= nt;
+        }
+        return false;
+    }
+    ci32_t* const split = ( ci32_t* )s.split;
+    for( int part = 0; part < 2; ++part )
+        for( int k = qb; k < qe; ++k )
+        {
+            int const sd = qseeds[ k ];
+            int const n = split[ 2 * sd + part ];
+            int const nt = ( n + TILE - 1 ) / TILE;
+            if( t < nt )
+            {
+                cptr_t const w = ( cptr_t )s.seeds + 25 * sd;
+                seed = sd; count = n; tile = t;
+                list = ( const int32_t* )w[ 24 ] + ( part == 0 ? 0 : ( int64_t )w[ 0 ] - n );
+                return true;
+            }
+            t -= nt;
+        }
+    return false;
+}
+
 // Tile loop of the persistent track kernels (HEAVY: lattices that also
 // hold the maps outside the lean set).
 template< bool HEAVY >
 ST_DEV void batch_track_tiles( const st_batch_sched_t& s, int64_t until_turn, int64_t max_turns,
                                int32_t* status )
 {
-    ci32_t* const counts = ( ci32_t* )s.counts;
-    ci32_t* const qseeds = ( ci32_t* )s.queue_seeds;
     ci32_t* const qbegin = ( ci32_t* )s.queue_begin;
     int const nq = s.n_queues;
     int const q0 = ( int )( blockIdx.x % ( unsigned )nq );
+    bool const one_tile = s.split != nullptr;
+    WavePrio const wp = launch_priority< 64 >( s, max_turns );
     for( int qi = 0; qi < nq; ++qi )
     {
         int const q = ( q0 + qi ) % nq;
@@ -118,31 +247,26 @@ ST_DEV void batch_track_tiles( const st_batch_sched_t& s, int64_t until_turn, in
             int t = 0;
             if( __lane_id() == 0 ) t = atomicAdd( &s.queue_next[ q ], 1 );
             t = __builtin_amdgcn_readfirstlane( __shfl( t, 0 ) );
-            int seed = -1;
-            for( int k = qb; k < qe; ++k )
-            {
-                int const sd = qseeds[ k ];
-                int const nt = ( counts[ 2 * sd ] + 63 ) >> 6;
-                if( t < nt ) { seed = sd; break; }
-                t -= nt;
-            }
-            if( seed < 0 ) break;                    // this queue is drained
+            int seed = 0, tile = 0;
+            int32_t count = 0;
+            const int32_t* list = nullptr;
+            if( !find_tile< 64 >( s, qb, qe, t, seed, list, count, tile ) ) break;   // queue drained
             if constexpr( HEAVY )
             {
                 // the particle set is read from the seed descriptor where
                 // it is used (its first 22 words are the set's view)
                 cptr_t const w = ( cptr_t )s.seeds + 25 * seed;
-                track_lean_lane< true >( PDesc{ w }, ( const int32_t* )w[ 24 ], counts[ 2 * seed ],
-                                         ( int64_t )t * 64 + __lane_id(), ( cptr_t )w[ 22 ],
-                                         ( int64_t )w[ 23 ], until_turn, max_turns, status );
+                track_lean_lane< true >( PDesc{ w }, list, count, ( int64_t )tile * 64 + __lane_id(),
+                                         ( cptr_t )w[ 22 ], ( int64_t )w[ 23 ], until_turn, max_turns,
+                                         status, wp );
             }
             else
             {
                 SeedView const S = load_seed( s.seeds, seed );
-                track_lean_lane< false >( PDirect{ S.P }, S.list, counts[ 2 * seed ],
-                                          ( int64_t )t * 64 + __lane_id(), S.lat, S.n_elem,
-                                          until_turn, max_turns, status );
+                track_lean_lane< false >( PDirect{ S.P }, list, count, ( int64_t )tile * 64 + __lane_id(),
+                                          S.lat, S.n_elem, until_turn, max_turns, status, wp );
             }
+            if( one_tile ) return;
         }
     }
 }
@@ -175,11 +299,11 @@ __global__ void __launch_bounds__( 256, ST_LEAN2_WAVES_PER_EU )
 KN( st_k_batch_track2 )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
                          int32_t* status )
 {
-    ci32_t* const counts = ( ci32_t* )s.counts;
-    ci32_t* const qseeds = ( ci32_t* )s.queue_seeds;
     ci32_t* const qbegin = ( ci32_t* )s.queue_begin;
     int const nq = s.n_queues;
     int const q0 = ( int )( blockIdx.x % ( unsigned )nq );
+    bool const one_tile = s.split != nullptr;
+    WavePrio const wp = launch_priority< 128 >( s, max_turns );
     for( int qi = 0; qi < nq; ++qi )
     {
         int const q = ( q0 + qi ) % nq;
@@ -189,18 +313,14 @@ KN( st_k_batch_track2 )( st_batch_sched_t s, int64_t until_turn, int64_t max_tur
             int t = 0;
             if( __lane_id() == 0 ) t = atomicAdd( &s.queue_next[ q ], 1 );
             t = __builtin_amdgcn_readfirstlane( __shfl( t, 0 ) );
-            int seed = -1;
-            for( int k = qb; k < qe; ++k )
-            {
-                int const sd = qseeds[ k ];
-                int const nt = ( counts[ 2 * sd ] + 127 ) >> 7;
-                if( t < nt ) { seed = sd; break; }
-                t -= nt;
-            }
-            if( seed < 0 ) break;                    // this queue is drained
+            int seed = 0, tile = 0;
+            int32_t count = 0;
+            const int32_t* list = nullptr;
+            if( !find_tile< 128 >( s, qb, qe, t, seed, list, count, tile ) ) break;  // queue drained
             SeedView const S = load_seed( s.seeds, seed );
-            track_lean_lane2( S.P, S.list, counts[ 2 * seed ], ( int64_t )t * 128, S.lat,
-                              S.n_elem, until_turn, max_turns, status );
+            track_lean_lane2( S.P, list, count, ( int64_t )tile * 128, S.lat, S.n_elem, until_turn,
+                              max_turns, status, wp );
+            if( one_tile ) return;
         }
     }
 }
@@ -220,6 +340,21 @@ extern "C" hipError_t KN( st_launch_batch_select )( const st_batch_sched_t* s,
     unsigned const gx = ( unsigned )( ( max_particles + 255 ) / 256 );
     hipLaunchKernelGGL( KN( st_k_batch_select ), dim3( gx, ( unsigned )s->n_seeds ), dim3( 256 ),
                         0, stream, *s, until_turn );
+    return hipGetLastError();
+}
+
+extern "C" hipError_t KN( st_launch_batch_select_split )( const st_batch_sched_t* s,
+    int64_t max_particles, int64_t until_turn, hipStream_t stream )
+{
+    if( max_particles <= 0 || s->n_seeds <= 0 ) return hipSuccess;
+    if( s->split == nullptr || s->gmin == nullptr ) return hipErrorInvalidValue;
+    unsigned const gx = ( unsigned )( ( max_particles + 255 ) / 256 );
+    hipLaunchKernelGGL( KN( st_k_batch_minturn ), dim3( gx, ( unsigned )s->n_seeds ), dim3( 256 ),
+                        0, stream, *s, until_turn );
+    hipError_t const e = hipGetLastError();
+    if( e != hipSuccess ) return e;
+    hipLaunchKernelGGL( KN( st_k_batch_select_split ), dim3( gx, ( unsigned )s->n_seeds ),
+                        dim3( 256 ), 0, stream, *s, until_turn );
     return hipGetLastError();
 }
 

@@ -11,11 +11,19 @@
 //           most K turns, steal from other queues when theirs is drained;
 // optionally captured once into a HIP graph.  Everything is enqueued on
 // the caller's stream; nothing here synchronises except select_sync().
+//
+// run_balanced(U) is the alternative for one request on a host thread that
+// may wait: every launch gives each resident wave at most one tile, taken
+// from the particles furthest behind first, and the host reads the counts
+// back between launches.  With T tiles and W resident waves, W < T < 2W,
+// the persistent form ends every chunk with T - W tiles on a nearly idle
+// chip; here every launch but the last of a request fills the W waves.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <climits>
 #include <cstdio>
 #include <cstdlib>
@@ -54,14 +62,16 @@ struct Config
     int ppl = 1;                 // particles per lane (1 or 2)
     bool heavy = false;          // lattices hold maps outside the lean set
                                  // (set by the owner, one particle per lane)
+    bool balance = false;        // run_balanced() available (split lists)
+    bool wave_prio = true;       // waves with more turns left issue first
 
     std::string str() const
     {
         char b[ 192 ];
         std::snprintf( b, sizeof( b ), "fma=%d block=%d waves_per_cu=%d queues=%d "
-                       "turns_per_launch=%lld graph=%d ppl=%d%s", fma ? 1 : 0, block, waves_per_cu,
-                       queues, ( long long )turns_per_launch, graph ? 1 : 0, ppl,
-                       heavy ? " heavy" : "" );
+                       "turns_per_launch=%lld graph=%d ppl=%d wave_prio=%d%s%s", fma ? 1 : 0, block,
+                       waves_per_cu, queues, ( long long )turns_per_launch, graph ? 1 : 0, ppl,
+                       wave_prio ? 1 : 0, heavy ? " heavy" : "", balance ? " balance" : "" );
         return b;
     }
 
@@ -87,6 +97,7 @@ struct Config
         }
         else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
         else if( k == "graph" ) graph = iv != 0;
+        else if( k == "wave_prio" ) wave_prio = iv != 0;
         else if( k == "ppl" )
         {
             if( iv != 1 && iv != 2 ) return fail( "ppl (particles per lane) must be 1 or 2" );
@@ -128,6 +139,9 @@ struct BatchRunner
     int32_t* d_qbegin = nullptr;
     int32_t* d_qnext = nullptr;        // [8]
     int32_t* h_counts = nullptr;       // pinned [2 * S]
+    int32_t* d_split = nullptr;        // balanced form: [2 * S] front / back counts, then gmin
+    int32_t* h_split = nullptr;        // pinned [2 * S + 1]
+    unsigned long long* d_progress = nullptr;   // balanced form: turns done in a launch
     hipGraphExec_t gexec = nullptr;
     std::string gkey;
 
@@ -141,10 +155,13 @@ struct BatchRunner
         if( gexec ) ( void )hipGraphExecDestroy( gexec );
         gexec = nullptr; gkey.clear();
         for( void* p : { ( void* )d_seeds, ( void* )d_lists, ( void* )d_counts, ( void* )d_qseeds,
-                         ( void* )d_qbegin, ( void* )d_qnext } )
+                         ( void* )d_qbegin, ( void* )d_qnext, ( void* )d_split, ( void* )d_progress } )
             if( p ) ( void )hipFree( p );
         if( h_counts ) ( void )hipHostFree( h_counts );
+        if( h_split ) ( void )hipHostFree( h_split );
         d_seeds = nullptr; d_lists = d_counts = d_qseeds = d_qbegin = d_qnext = h_counts = nullptr;
+        d_split = h_split = nullptr;
+        d_progress = nullptr;
         n_part.clear(); n_elem.clear(); total_particles = max_particles = 0;
     }
 
@@ -177,6 +194,13 @@ struct BatchRunner
             !hip_ok( hipHostMalloc( &h_counts, std::max< int64_t >( S, 1 ) * 2 * sizeof( int32_t ) ),
                      "hipHostMalloc counts" ) ||
             !hip_ok( hipMalloc( &d_qnext, 8 * sizeof( int32_t ) ), "hipMalloc queues" ) ) return false;
+        if( cfg.balance &&
+            ( !hip_ok( hipMalloc( &d_split, ( 2 * std::max< int64_t >( S, 1 ) + 1 ) * sizeof( int32_t ) ),
+                       "hipMalloc split counts" ) ||
+              !hip_ok( hipHostMalloc( &h_split, ( 2 * std::max< int64_t >( S, 1 ) + 1 ) * sizeof( int32_t ) ),
+                       "hipHostMalloc split counts" ) ||
+              !hip_ok( hipMalloc( &d_progress, sizeof( unsigned long long ) ), "hipMalloc progress" ) ) )
+            return false;
         int64_t list_off = 0;
         for( int64_t s = 0; s < S; ++s )
         {
@@ -235,7 +259,18 @@ struct BatchRunner
         st_batch_sched_t s{};
         s.seeds = d_seeds; s.counts = d_counts; s.queue_seeds = d_qseeds;
         s.queue_begin = d_qbegin; s.queue_next = d_qnext;
+        s.split = nullptr; s.gmin = nullptr; s.progress = nullptr;
         s.n_seeds = ( int32_t )n_seeds(); s.n_queues = cfg.queues;
+        s.wave_prio = cfg.wave_prio ? 1 : 0;
+        return s;
+    }
+
+    st_batch_sched_t sched_split() const
+    {
+        st_batch_sched_t s = sched();
+        s.split = d_split;
+        s.gmin = d_split + 2 * n_seeds();
+        s.progress = d_progress;
         return s;
     }
 
@@ -252,6 +287,8 @@ struct BatchRunner
         double const k = 5.0e11 / work;
         return k >= 1.0e9 ? ( int64_t )1000000000 : ( k < 1.0 ? 1 : ( int64_t )k );
     }
+
+    int64_t waves() const { return ( int64_t )grid() * ( cfg.block / 64 ); }
 
     int grid() const
     {
@@ -306,6 +343,116 @@ struct BatchRunner
         }
         if( n_active == 0 ) min_turn = until_turn;
         return true;
+    }
+
+    // Balanced form: turns per launch for T tiles at r turns to go.  The
+    // launches of a request are about T r / (W K); K is chosen so that
+    // there are at least eight (when T > W), which keeps the last,
+    // partly filled one short, and at most chunk_turns() of W tiles.
+    int64_t balanced_turns( int64_t tiles, int64_t r, int64_t n_active ) const
+    {
+        if( cfg.turns_per_launch > 0 ) return cfg.turns_per_launch;
+        int64_t const W = std::max< int64_t >( waves(), 1 );
+        if( tiles <= W ) return std::min( chunk_turns( n_active ), std::max< int64_t >( r, 1 ) );
+        int64_t const kmax = chunk_turns( W * 64 * cfg.ppl );
+        double const work = ( double )tiles * ( double )r / ( double )W;    // in launch-turns
+        int64_t const L = std::max< int64_t >( 8, ( int64_t )std::ceil( work / ( double )kmax ) );
+        int64_t const k = ( int64_t )std::ceil( work / ( double )L );
+        return std::max< int64_t >( 1, std::min( k, kmax ) );
+    }
+
+    // Balanced form: reset and the two select passes, enqueued.
+    bool enqueue_select_split( int64_t until_turn, hipStream_t stream ) const
+    {
+        st_batch_sched_t const s = sched_split();
+        return hip_ok( cfg.fma ? st_launch_batch_reset_fma( &s, stream )
+                               : st_launch_batch_reset_exact( &s, stream ), "launch batch reset" ) &&
+               hip_ok( cfg.fma ? st_launch_batch_select_split_fma( &s, max_particles, until_turn, stream )
+                               : st_launch_batch_select_split_exact( &s, max_particles, until_turn,
+                                                                     stream ),
+                       "launch batch select" );
+    }
+
+    // Balanced launches with fewer tiles than resident waves get a grid of
+    // about one wave per tile, which the dispatcher spreads evenly over the
+    // CUs; a full grid would leave some SIMDs with 8 of the waves that have
+    // a tile and others with few.  A grid smaller than the tiles only
+    // defers tiles to the next launch.
+    int grid_for_tiles( int64_t tiles ) const
+    {
+        int64_t const per_block = cfg.block / 64;
+        int64_t const g = ( tiles + per_block - 1 ) / per_block;
+        return ( int )std::max< int64_t >( cfg.queues, std::min< int64_t >( grid(), g ) );
+    }
+
+    bool enqueue_track_split( int64_t until_turn, int64_t k, int32_t* d_status, int g,
+                              hipStream_t stream ) const
+    {
+        st_batch_sched_t const s = sched_split();
+        hipError_t const e = cfg.fma
+            ? st_launch_batch_track_fma( &s, until_turn, k, d_status, cfg.block, g, cfg.ppl,
+                                         cfg.heavy ? 1 : 0, stream )
+            : st_launch_batch_track_exact( &s, until_turn, k, d_status, cfg.block, g, cfg.ppl,
+                                           cfg.heavy ? 1 : 0, stream );
+        return hip_ok( e, "launch batch track" );
+    }
+
+    // Balanced form, one select pass read back (synchronises the stream):
+    // particles with turns to go, the smallest at_turn, and the tiles of
+    // the split lists.
+    bool select_split_sync( int64_t until_turn, hipStream_t stream, int64_t& n_active,
+                            int64_t& min_turn, int64_t& tiles )
+    {
+        int64_t const S = n_seeds();
+        n_active = 0; tiles = 0; min_turn = until_turn;
+        if( S == 0 ) return true;
+        if( d_split == nullptr ) return fail( "balanced scheduling was not set up" );
+        int64_t const tile = 64 * cfg.ppl;
+        if( !enqueue_select_split( until_turn, stream ) ||
+            !hip_ok( hipMemcpyAsync( h_split, d_split, ( 2 * S + 1 ) * sizeof( int32_t ),
+                                     hipMemcpyDeviceToHost, stream ), "counts D2H" ) ||
+            !hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" ) ) return false;
+        for( int64_t k = 0; k < 2 * S; ++k )
+        {
+            n_active += h_split[ k ];
+            tiles += ( h_split[ k ] + tile - 1 ) / tile;
+        }
+        if( n_active > 0 ) min_turn = h_split[ 2 * S ];
+        return true;
+    }
+
+    // until_turn for every seed, balanced (see the top of this file); the
+    // OR of statuses into *d_status.  The counts are read back once per
+    // group of launches: a group is as many launches as the tiles need
+    // without losses (every particle ceil(r / K) tile visits, W visits per
+    // launch), each preceded by its select passes on the device; a group
+    // that leaves particles behind (partly filled tiles) is followed by
+    // another.  Every launch advances at least one tile by a turn or loses
+    // its particles, so the loop ends; the bound on launches only turns a
+    // fault into an error.
+    bool run_balanced( int64_t until_turn, int32_t* d_status, hipStream_t stream )
+    {
+        int64_t n_active = 0, min_turn = 0, tiles = 0, k = 0, limit = 0, launches = 0;
+        int64_t const W = std::max< int64_t >( waves(), 1 );
+        for( int64_t group = 0;; ++group )
+        {
+            if( !select_split_sync( until_turn, stream, n_active, min_turn, tiles ) ) return false;
+            if( n_active == 0 ) return true;
+            if( group == 0 )
+            {
+                k = balanced_turns( tiles, until_turn - min_turn, n_active );
+                limit = 16 + 4 * ( ( tiles + W - 1 ) / W ) * ( ( until_turn - min_turn + k - 1 ) / k );
+            }
+            int64_t const v = ( until_turn - min_turn + k - 1 ) / k;     // visits per particle
+            int64_t const n = std::max( v, ( tiles * v + W - 1 ) / W );
+            int const g = grid_for_tiles( tiles );
+            for( int64_t j = 0; j < n; ++j, ++launches )
+            {
+                if( launches >= limit ) return fail( "balanced tracking made no progress" );
+                if( j > 0 && !enqueue_select_split( until_turn, stream ) ) return false;
+                if( !enqueue_track_split( until_turn, k, d_status, g, stream ) ) return false;
+            }
+        }
     }
 
     // C chunks of K turns towards until_turn; the OR of statuses into

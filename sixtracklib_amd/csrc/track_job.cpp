@@ -178,6 +178,9 @@ struct NS(TrackJobBase)
     // XCD the waves of the first XCD would take every tile of a small set
     int queues = 1;
     int ppl = 1;                         // particles per lane of the lean until path
+    bool balance = true;                 // one tile per wave and launch, furthest behind first
+    bool wave_prio = true;               // waves with more turns left issue first
+    int waves_per_cu = 0;                // resident waves per CU (0: the kernel's occupancy)
     st_batch::BatchRunner runner;        // scheduling of the lean until path
     bool runner_ready = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -476,7 +479,11 @@ struct NS(TrackJobBase)
     // next.  Every particle is tracked turn by turn on its own, so the
     // results are those of one launch.  Lattices with maps outside the
     // lean set run the heavy variant of the same loop (those records out
-    // of line).  compact=0 runs one launch with one lane per particle.
+    // of line).  By default (balance=1) each launch gives every resident
+    // wave one tile, the particles furthest behind first, and the host
+    // reads the counts back between launches; balance=0 hands every wave
+    // tiles until none are left in each chunk.  compact=0 runs one launch
+    // with one lane per particle.
     bool ensure_runner()
     {
         if( runner_ready ) return true;
@@ -487,9 +494,12 @@ struct NS(TrackJobBase)
         runner.cfg.queues = queues;
         runner.cfg.heavy = lat_full;
         runner.cfg.ppl = lat_full ? 1 : ppl;
+        runner.cfg.balance = balance;
+        runner.cfg.wave_prio = wave_prio;
         // resident waves per CU: 8 per SIMD with one particle per lane, 5
         // with two (the kernels' register budgets)
         runner.cfg.waves_per_cu = runner.cfg.ppl == 2 ? 20 : 32;
+        if( waves_per_cu > 0 ) runner.cfg.waves_per_cu = std::min( waves_per_cu, runner.cfg.waves_per_cu );
         runner.cfg.graph = false;
         std::vector< st_batch_seed_t > seeds;
         for( const st_dev_particles_t& v : views )
@@ -509,6 +519,8 @@ struct NS(TrackJobBase)
         if( compact && n_elem > 0 )
         {
             int64_t n_active = 0, min_turn = 0;
+            if( balance ) return end_status( ensure_runner() &&
+                                             runner.run_balanced( until_turn, d_status, stream ) );
             ok = ensure_runner() && runner.select_sync( until_turn, stream, n_active, min_turn );
             if( ok && n_active > 0 )
             {
@@ -593,6 +605,9 @@ struct NS(TrackJobBase)
             else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
             else if( k == "queues" ) queues = ( iv == 8 ) ? 8 : 1;
             else if( k == "ppl" ) ppl = ( iv == 2 ) ? 2 : 1;
+            else if( k == "balance" ) balance = iv != 0;
+            else if( k == "wave_prio" ) wave_prio = iv != 0;
+            else if( k == "waves_per_cu" ) waves_per_cu = ( iv >= 1 && iv <= 32 ) ? ( int )iv : 0;
         }
     }
 };

@@ -111,17 +111,73 @@ ST_DEV cptr_t rec_at( cptr_t lat, uint32_t off )
     return ( cptr_t )( ( const __attribute__( ( address_space( 4 ) ) ) char* )lat + off );
 }
 
+// Issue priority of a wave in a persistent launch.  Among waves of equal
+// priority a SIMD issues the oldest first, so with equal priorities the
+// waves of a SIMD progress unequally, and the last of them finish with few
+// waves beside them (latency-bound).  Two rules, both wave-uniform and
+// applied at turn ends:
+//   progress == nullptr: by the turns the wave has left of the k the launch
+//     gives a tile, in four bands, more left = higher (the waves that will
+//     finish last, a tile started late or a slow one, go first);
+//   progress != nullptr: every wave adds its turns to *progress, and a wave
+//     behind the mean of the n_waves waves of the launch goes before one
+//     ahead of it (the waves stay within about a turn of each other).
+// k <= 0 leaves the priority alone.
+struct WavePrio
+{
+    unsigned long long* progress = nullptr;
+    int64_t n_waves = 0;
+    int64_t k = 0;
+};
+
+ST_DEV void set_wave_priority( int p )
+{
+    if( p >= 3 ) __builtin_amdgcn_s_setprio( 3 );
+    else if( p == 2 ) __builtin_amdgcn_s_setprio( 2 );
+    else if( p == 1 ) __builtin_amdgcn_s_setprio( 1 );
+    else __builtin_amdgcn_s_setprio( 0 );
+}
+
+// at the start of a tile (done = 0) and after each of its turns
+ST_DEV void wave_priority( const WavePrio& wp, int64_t done )
+{
+    if( wp.k <= 0 ) return;
+    if( wp.progress == nullptr )
+    {
+        set_wave_priority( ( int )( ( 4 * ( wp.k - done ) - 1 ) / wp.k ) );
+        return;
+    }
+    if( done == 0 ) { set_wave_priority( 2 ); return; }
+    unsigned long long v = 0;
+    if( __lane_id() == 0 ) v = atomicAdd( wp.progress, 1ull ) + 1ull;
+    v = __shfl( v, 0 );
+    int64_t const g = ( int64_t )( ( uint64_t )( uint32_t )__builtin_amdgcn_readfirstlane( ( int )( uint32_t )v ) |
+                                   ( ( uint64_t )( uint32_t )__builtin_amdgcn_readfirstlane( ( int )( v >> 32 ) ) << 32 ) );
+    int64_t const d = g - wp.n_waves * done;                 // > 0: behind the mean
+    set_wave_priority( d > wp.n_waves ? 3 : ( d > 0 ? 2 : ( d > -wp.n_waves ? 1 : 0 ) ) );
+}
+
+// a wave leaving its tile after done of k turns (its particles lost or at
+// their last turn) counts the rest as done, so that the mean stays that of
+// the waves still tracking
+ST_DEV void wave_priority_leave( const WavePrio& wp, int64_t done )
+{
+    if( wp.k <= 0 || wp.progress == nullptr || done >= wp.k ) return;
+    if( __lane_id() == 0 ) atomicAdd( wp.progress, ( unsigned long long )( wp.k - done ) );
+}
+
 // One lane of a lean until-turn launch (HEAVY: the lattice may also hold
 // the other maps, see lean_other).
 //   P          particle set; particle i = list[tid] with a list, else tid
 //   n_lanes    number of valid tid (list length, or P.n without a list)
 //   lat        packed lean lattice of n_elem records (END-terminated)
 //   max_turns  turns this launch may advance a lane (<= 0: until_turn)
+//   wp         issue priority of the wave (wave_priority)
 // The OR of the per-particle statuses goes to *status_out.
 template< bool HEAVY = false, class PA = PDirect >
 ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
                              int64_t tid, cptr_t lat, int64_t n_elem, int64_t until_turn,
-                             int64_t max_turns, int32_t* status_out )
+                             int64_t max_turns, int32_t* status_out, const WavePrio& wp = WavePrio() )
 {
     bool const in_range = tid < n_lanes;
     int64_t const i = ( list != nullptr && in_range ) ? ( int64_t )list[ tid ] : tid;
@@ -149,6 +205,8 @@ ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
     // A lane leaves ``act`` once, after its final state has been stored.  A
     // wave whose lanes are all lost finishes the turn without effect.
     uint64_t act = ballot( active0 );
+    int64_t turns_done = 0;                   // by this wave, in this call
+    wave_priority( wp, 0 );
     while( act != 0u )
     {
         uint32_t off = 0;
@@ -214,7 +272,10 @@ ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
             if( lane_in( fin ) ) store_lane_state( pa.view(), i, L, 1, start, at_turn );
             act &= ~fin;
         }
+        ++turns_done;
+        if( act != 0u ) wave_priority( wp, turns_done );
     }
+    wave_priority_leave( wp, turns_done );
     if( in_range && status != 0 ) atomicOr( status_out, status );
 }
 
@@ -227,7 +288,7 @@ ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
 // per lane (the Horner recurrences in particular).
 ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, int64_t n_lanes,
                               int64_t base, cptr_t lat, int64_t n_elem, int64_t until_turn,
-                              int64_t max_turns, int32_t* status_out )
+                              int64_t max_turns, int32_t* status_out, const WavePrio& wp = WavePrio() )
 {
     constexpr int K = 2;
     int64_t i[ K ], at_turn[ K ], start[ K ], turn_stop[ K ];
@@ -257,6 +318,8 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
         act[ k ] = ballot( active0 );
     }
 
+    int64_t turns_done = 0;
+    wave_priority( wp, 0 );
     while( ( act[ 0 ] | act[ 1 ] ) != 0u )
     {
         uint32_t off = 0;
@@ -348,7 +411,10 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
                 act[ k ] &= ~fin;
             }
         }
+        ++turns_done;
+        if( ( act[ 0 ] | act[ 1 ] ) != 0u ) wave_priority( wp, turns_done );
     }
+    wave_priority_leave( wp, turns_done );
     if( status != 0 ) atomicOr( status_out, status );
 }
 
