@@ -26,6 +26,8 @@
 //     counter and is only materialised at losses, output stores and the end.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "track_device.h"
 #include "track_lean.h"
 #include "track_launch.h"
@@ -388,9 +390,37 @@ extern "C" hipError_t KN( st_launch_track_ebe )( const st_dev_particles_t* P,
     a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
     a.cfg = *cfg; a.ebe_cap = cap; a.O = *O; a.status_out = status;
     a.ebe_slots = cfg->num_particles_to_store > 0 ? cap / cfg->num_particles_to_store : 0;
-    hipLaunchKernelGGL( KN( st_k_track_ebe ), dim3( grid_for( P->n, block ) ), dim3( block ),
-                        0, stream, *P, a );
-    return hipGetLastError();
+    // The record stores bound this kernel, and a SIMD holding two of its
+    // waves takes twice as long as one holding one: the store rate of a
+    // SIMD does not grow with its waves.  So the set is tracked in rounds
+    // of one wave per SIMD (64 particles x 4 SIMDs x CUs), one launch after
+    // the other, the rest in a last one; one wave per workgroup, so that
+    // the waves spread over all CUs.  (Equal rounds of fewer waves measured
+    // slower.)  Every particle is tracked on its own and its records' slots
+    // follow from its id, element and turn, so the rounds give the results
+    // of one launch.
+    ( void )block;
+    int n_cu = 0, dev = 0;
+    if( hipGetDevice( &dev ) != hipSuccess ||
+        hipDeviceGetAttribute( &n_cu, hipDeviceAttributeMultiprocessorCount, dev ) != hipSuccess ||
+        n_cu <= 0 ) n_cu = 1;
+    int64_t const per_round = 64 * 4 * ( int64_t )n_cu;                  // particles
+    for( int64_t begin = 0; begin < P->n; begin += per_round )
+    {
+        st_dev_particles_t sub = *P;
+        sub.n = std::min< int64_t >( per_round, P->n - begin );
+        sub.q0 += begin; sub.mass0 += begin; sub.beta0 += begin; sub.gamma0 += begin;
+        sub.p0c += begin; sub.s += begin; sub.x += begin; sub.y += begin; sub.px += begin;
+        sub.py += begin; sub.zeta += begin; sub.psigma += begin; sub.delta += begin;
+        sub.rpp += begin; sub.rvv += begin; sub.chi += begin; sub.charge_ratio += begin;
+        sub.particle_id += begin; sub.at_element += begin; sub.at_turn += begin;
+        sub.state += begin;
+        hipLaunchKernelGGL( KN( st_k_track_ebe ), dim3( grid_for( sub.n, 64 ) ), dim3( 64 ), 0,
+                            stream, sub, a );
+        hipError_t const e = hipGetLastError();
+        if( e != hipSuccess ) return e;
+    }
+    return hipSuccess;
 }
 
 extern "C" hipError_t KN( st_launch_track_line )( const st_dev_particles_t* P,

@@ -330,6 +330,83 @@ def test_elem_by_elem_reference_store_rolls_over_turns():
     assert np.any(got["at_turn"] >= 3)
 
 
+@pytest.mark.parametrize("n,turns,shuffle", [(130, 2, False), (129, 2, False),
+                                             (131, 3, False), (130, 2, True)])
+def test_elem_by_elem_store_layouts_match_oracle(n, turns, shuffle):
+    """The reference-size element-by-element store for particle counts and
+    ids that change how a wave's records lie in the store, against the
+    oracle's store bit for bit: 130 particles (neighbouring particles 8 and
+    9 lost at different elements), 129 (odd: consecutive passes of a wave
+    alternate between even and odd record offsets; the last wave holds one
+    particle), 131 over 3 turns (odd capacity: the field arrays alternate
+    between 16-byte and 8-byte alignment), and 130 with shuffled particle
+    ids (neighbouring lanes' records are not adjacent)."""
+    from sixtracklib_amd.trackjob import TrackJob
+    from sixtracklib_amd.particles import particles_in, Particles
+    eb = lattices.collective_ring(n_cells=8, n_sc=4)
+    E = eb.n_objects
+    pb = _collective_beam(n)
+    if shuffle:
+        pv = particles_in(pb)
+        type(pv).__dict__["particle_id"]._view(pv)[:] = np.random.default_rng(5).permutation(n)
+    ob = _copy(pb)
+    job = TrackJob(eb, pb, until_turn_elem_by_elem=turns, config_str="fma=0")
+    off = job.elem_by_elem_output_offset
+    job.track_elem_by_elem(turns)
+    job.collect()
+    got = _pdict(job.output, off)
+    assert len(got["x"]) == n * E * turns
+    ocb = CBuffer(max_slots=22 * n * E * turns + 64)
+    Particles(cbuffer=ocb, num_particles=n * E * turns)
+    cfg = oracle_lib.EbeConfig(order=0, num_particles_to_store=n, num_elements_to_store=E,
+                               num_turns_to_store=turns, min_particle_id=0, min_element_id=0,
+                               min_turn=0, max_particle_id=n - 1, max_element_id=E - 1,
+                               max_turn=turns - 1, is_rolling=1,
+                               out_store_addr=ocb.get_object_address(0))
+    assert oracle_lib.track_elem_by_elem(ob, eb, cfg, turns) == 0
+    assert_particles_equal(_pdict(pb), _pdict(ob), what="particles")
+    d = _pdict(ob)                                           # the pair (8, 9) breaks:
+    assert d["state"][9] == 0 and (d["state"][8] == 1 or     # 9 is lost before 8
+           (d["at_turn"][8], d["at_element"][8]) > (d["at_turn"][9], d["at_element"][9]))
+    assert_particles_equal(got, _pdict(ocb), what="store")
+
+
+def test_elem_by_elem_in_rounds_matches_oracle():
+    """A set larger than one wave per SIMD (70,000 particles; 65,536 fill
+    the 1,024 SIMDs of a 256-CU device) is tracked element by element in
+    rounds of one wave per SIMD, one launch after the other; records and
+    particles equal the oracle's bit for bit, losses at the aperture
+    included."""
+    from sixtracklib_amd.trackjob import TrackJob
+    from sixtracklib_amd.particles import particles_in, Particles
+    eb = CBuffer()
+    be.Drift(cbuffer=eb, length=0.7)
+    be.Multipole(cbuffer=eb, knl=[0.0, 0.02, 0.3], ksl=[0.0, 0.0, 0.05])
+    be.LimitRect(cbuffer=eb, min_x=-0.01, max_x=0.01, min_y=-0.01, max_y=0.01)
+    be.Drift(cbuffer=eb, length=1.3)
+    E, T, n = 4, 3, 70000
+    pb, _ = lattices.gaussian_beam(n, seed=8, sigma_x=4e-3, sigma_y=4e-3, sigma_px=2e-4,
+                                   sigma_py=2e-4)
+    ob = _copy(pb)
+    job = TrackJob(eb, pb, until_turn_elem_by_elem=T, config_str="fma=0")
+    off = job.elem_by_elem_output_offset
+    assert job.track_elem_by_elem(T) == 0
+    job.collect()
+    got = _pdict(job.output, off)
+    ocb = CBuffer(max_slots=22 * n * E * T + 64)
+    Particles(cbuffer=ocb, num_particles=n * E * T)
+    cfg = oracle_lib.EbeConfig(order=0, num_particles_to_store=n, num_elements_to_store=E,
+                               num_turns_to_store=T, min_particle_id=0, min_element_id=0,
+                               min_turn=0, max_particle_id=n - 1, max_element_id=E - 1,
+                               max_turn=T - 1, is_rolling=1,
+                               out_store_addr=ocb.get_object_address(0))
+    assert oracle_lib.track_elem_by_elem(ob, eb, cfg, T) == 0
+    want = _pdict(ob)
+    assert 1000 < np.count_nonzero(want["state"] == 0) < n - 1000
+    assert_particles_equal(_pdict(pb), want, what="particles")
+    assert_particles_equal(got, _pdict(ocb), what="store")
+
+
 @pytest.mark.parametrize("passes", [300, 407 * 4 - 5])
 def test_elem_by_elem_rolling_passes_keep_the_latest_records(passes):
     """Collective ring (BB6D, BB4D, space charge; 407 elements), 4 turns
