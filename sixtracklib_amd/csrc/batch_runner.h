@@ -345,20 +345,30 @@ struct BatchRunner
         return true;
     }
 
-    // Balanced form: turns per launch for T tiles at r turns to go.  The
-    // launches of a request are about T r / (W K); K is chosen so that
-    // there are at least eight (when T > W), which keeps the last,
-    // partly filled one short, and at most chunk_turns() of W tiles.
+    // Balanced form: turns per launch for T tiles at r turns to go.  With
+    // K turns a particle needs v = ceil(r / K) visits, and the request about
+    // ceil(T v / W) launches of K turns each, the last one part-filled, and
+    // each ends with about a turn of drain (waves a turn apart finishing).
+    // K is the one of at most chunk_turns() of W tiles that minimises
+    // launches x (K + 1) over v = vmin .. vmin + 32.
     int64_t balanced_turns( int64_t tiles, int64_t r, int64_t n_active ) const
     {
         if( cfg.turns_per_launch > 0 ) return cfg.turns_per_launch;
+        r = std::max< int64_t >( r, 1 );
         int64_t const W = std::max< int64_t >( waves(), 1 );
-        if( tiles <= W ) return std::min( chunk_turns( n_active ), std::max< int64_t >( r, 1 ) );
-        int64_t const kmax = chunk_turns( W * 64 * cfg.ppl );
-        double const work = ( double )tiles * ( double )r / ( double )W;    // in launch-turns
-        int64_t const L = std::max< int64_t >( 8, ( int64_t )std::ceil( work / ( double )kmax ) );
-        int64_t const k = ( int64_t )std::ceil( work / ( double )L );
-        return std::max< int64_t >( 1, std::min( k, kmax ) );
+        if( tiles <= W ) return std::min( chunk_turns( n_active ), r );
+        int64_t const kmax = std::min( chunk_turns( W * 64 * cfg.ppl ), r );
+        int64_t const vmin = ( r + kmax - 1 ) / kmax;
+        int64_t best_k = kmax, best_cost = INT64_MAX;
+        for( int64_t v = vmin; v <= std::min( vmin + 32, r ); ++v )
+        {
+            int64_t const k = ( r + v - 1 ) / v;
+            int64_t const visits = ( r + k - 1 ) / k;
+            int64_t const launches = ( tiles * visits + W - 1 ) / W;
+            int64_t const cost = launches * ( k + 1 );
+            if( cost < best_cost ) { best_cost = cost; best_k = k; }
+        }
+        return best_k;
     }
 
     // Balanced form: reset and the two select passes, enqueued.
