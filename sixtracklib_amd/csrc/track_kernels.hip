@@ -76,7 +76,11 @@ ST_DEV void store_record_narrow( const st_dev_particles_t& O, int64_t k, double 
     const Lane& L, const LaneConst& C, int64_t pid, int64_t at_element, int64_t at_turn,
     bool constants )
 {
-    uint32_t const b = static_cast< uint32_t >( k ) * 8u;
+    // the offset is made opaque here, so that each store forms its address
+    // from the field's base (uniform) and this 32-bit offset at the store,
+    // rather than the loop carrying one running 64-bit address per field
+    uint32_t b = static_cast< uint32_t >( k ) * 8u;
+    asm volatile( "" : "+v"( b ) );
     auto d = [ b ]( double* base ) -> double&
     { return *reinterpret_cast< double* >( reinterpret_cast< char* >( base ) + b ); };
     auto n = [ b ]( int64_t* base ) -> int64_t&
@@ -117,18 +121,24 @@ ST_DEV void step_other( unsigned op, unsigned aux, cptr_t e, Lane& L, const Lane
     }
     if( FULL )
     {
+        // The maps below are out of line and take the particle by
+        // reference: they get copies, so that the caller's state (L, C)
+        // never has its address taken and stays in registers (an escaping
+        // L or C would live in scratch for the whole element loop).
+        Lane t = L;
+        LaneConst const ct = C;
         switch( op )
         {
         case ST_OP_MONITOR:
-            if( monitor( e, P, i, L, C, pid, at_element, at_turn, active ) )
+            if( monitor( e, P, i, t, ct, pid, at_element, at_turn, active ) )
             { ok = false; fail = true; }
             return;
-        case ST_OP_BB4D: bb4d( L, C, e ); return;
-        case ST_OP_BB6D: bb6d( L, C, e ); return;
-        case ST_OP_SC_COASTING: sc_coasting( L, C, e ); return;
-        case ST_OP_SC_QGAUSS: sc_qgauss( L, C, e ); return;
-        case ST_OP_SC_INTERPOL: sc_interpol( L, C, e ); return;
-        case ST_OP_RF_MULTIPOLE: rf_multipole( L, C, e, aux ); return;
+        case ST_OP_BB4D: bb4d( t, ct, e ); L = t; return;
+        case ST_OP_BB6D: bb6d( t, ct, e ); L = t; return;
+        case ST_OP_SC_COASTING: sc_coasting( t, ct, e ); L = t; return;
+        case ST_OP_SC_QGAUSS: sc_qgauss( t, ct, e ); L = t; return;
+        case ST_OP_SC_INTERPOL: sc_interpol( t, ct, e ); L = t; return;
+        case ST_OP_RF_MULTIPOLE: rf_multipole( t, ct, e, aux ); L = t; return;
         case ST_OP_FAIL: fail = true; return;           // disabled map
         default: break;
         }
@@ -335,7 +345,11 @@ KN( st_k_track_until_lean )( st_dev_particles_t P, Args a )
     track_until_lean( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
 }
 
-__global__ void __launch_bounds__( 256 ) KN( st_k_track_ebe )( st_dev_particles_t P, Args a )
+// One wave per workgroup and, in the launcher's rounds, one per SIMD: the
+// whole register file is this wave's, so the full map set keeps its state
+// and the record fields in registers (no scratch spills on the store path).
+__global__ void __launch_bounds__( 64 ) __attribute__( ( amdgpu_waves_per_eu( 1, 1 ) ) )
+KN( st_k_track_ebe )( st_dev_particles_t P, Args a )
 {
     track_body< MODE_EBE, true >( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
 }
