@@ -27,6 +27,12 @@
  *                         (128-particle tiles; results identical)
  *   wave_prio=0|1         waves with more turns left in their tile get
  *                         issue priority (1; results identical)
+ *   balance=0|1           0: chunks of turns, every wave taking tiles until
+ *                         none are left (graph-captured); 1: launches of one
+ *                         tile per resident wave, the particles furthest
+ *                         behind first, the host reading the counts back
+ *                         between launch groups (no graph; results
+ *                         identical)
  * NS(TrackBatch_autotune) picks particles per lane, block, waves_per_cu
  * and queues by timing short trials on the batch itself (state restored
  * afterwards).

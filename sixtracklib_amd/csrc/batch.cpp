@@ -205,6 +205,26 @@ struct NS(TrackBatch)
         return *h_status;
     }
 
+    // The balanced form (batch_runner.h, config token balance=1): launches
+    // of one tile per resident wave, the host reading the counts back
+    // between launch groups (no graph).
+    int32_t track_until_balanced( int64_t until_turn )
+    {
+        *h_status = 0;
+        if( !hip_ok( hipMemcpyAsync( d_status, h_status, sizeof( int32_t ), hipMemcpyHostToDevice,
+                                     stream ), "status reset" ) ||
+            !hip_ok( hipEventRecord( ev0, stream ), "hipEventRecord" ) ||
+            !run.run_balanced( until_turn, d_status, stream ) ||
+            !hip_ok( hipEventRecord( ev1, stream ), "hipEventRecord" ) ||
+            !hip_ok( hipMemcpyAsync( h_status, d_status, sizeof( int32_t ), hipMemcpyDeviceToHost,
+                                     stream ), "status D2H" ) ||
+            !hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" ) ) return -1;
+        float ms = 0.0f;
+        last_ms = ( hipEventElapsedTime( &ms, ev0, ev1 ) == hipSuccess ) ? ms : -1.0f;
+        if( until_turn > min_turn ) min_turn = until_turn;
+        return *h_status;
+    }
+
     // Candidate (particles per lane, block, waves_per_cu, queues) timed on
     // trial_turns turns of the batch itself; the particle images are saved
     // first and restored after every trial, so the batch state is
@@ -238,7 +258,8 @@ struct NS(TrackBatch)
                     run.cfg.graph = false;
                     if( !run.build_queues( stream ) ) { ok = false; break; }
                     min_turn = t0;
-                    int32_t const st = track_until( t0 + trial_turns, trial_turns );
+                    int32_t const st = run.cfg.balance ? track_until_balanced( t0 + trial_turns )
+                                                       : track_until( t0 + trial_turns, trial_turns );
                     ok = st >= -1 && last_ms >= 0.0f;
                     if( ok && ( best_ms < 0.0f || last_ms < best_ms ) ) { best_ms = last_ms; best = run.cfg; }
                     ok = ok && hip_ok( hipMemcpyAsync( d_img, backup, total_img, hipMemcpyDeviceToDevice,
@@ -277,7 +298,9 @@ void NS(TrackBatch_delete)( NS(TrackBatch)* batch ) { delete batch; }
 
 NS(track_status_t) NS(TrackBatch_track_until)( NS(TrackBatch)* batch, int64_t until_turn )
 {
-    return batch ? batch->track_until( until_turn ) : -1;
+    if( batch == nullptr ) return -1;
+    return batch->run.cfg.balance ? batch->track_until_balanced( until_turn )
+                                  : batch->track_until( until_turn );
 }
 
 int NS(TrackBatch_collect)( NS(TrackBatch)* batch ) { return ( batch && batch->collect() ) ? 0 : -1; }

@@ -98,6 +98,7 @@ struct Config
         else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
         else if( k == "graph" ) graph = iv != 0;
         else if( k == "wave_prio" ) wave_prio = iv != 0;
+        else if( k == "balance" ) balance = iv != 0;
         else if( k == "ppl" )
         {
             if( iv != 1 && iv != 2 ) return fail( "ppl (particles per lane) must be 1 or 2" );
@@ -194,12 +195,12 @@ struct BatchRunner
             !hip_ok( hipHostMalloc( &h_counts, std::max< int64_t >( S, 1 ) * 2 * sizeof( int32_t ) ),
                      "hipHostMalloc counts" ) ||
             !hip_ok( hipMalloc( &d_qnext, 8 * sizeof( int32_t ) ), "hipMalloc queues" ) ) return false;
-        if( cfg.balance &&
-            ( !hip_ok( hipMalloc( &d_split, ( 2 * std::max< int64_t >( S, 1 ) + 1 ) * sizeof( int32_t ) ),
-                       "hipMalloc split counts" ) ||
-              !hip_ok( hipHostMalloc( &h_split, ( 2 * std::max< int64_t >( S, 1 ) + 1 ) * sizeof( int32_t ) ),
-                       "hipHostMalloc split counts" ) ||
-              !hip_ok( hipMalloc( &d_progress, sizeof( unsigned long long ) ), "hipMalloc progress" ) ) )
+        // the balanced form's counts (a configuration may switch to it later)
+        if( !hip_ok( hipMalloc( &d_split, ( 2 * std::max< int64_t >( S, 1 ) + 1 ) * sizeof( int32_t ) ),
+                     "hipMalloc split counts" ) ||
+            !hip_ok( hipHostMalloc( &h_split, ( 2 * std::max< int64_t >( S, 1 ) + 1 ) * sizeof( int32_t ) ),
+                     "hipHostMalloc split counts" ) ||
+            !hip_ok( hipMalloc( &d_progress, sizeof( unsigned long long ) ), "hipMalloc progress" ) )
             return false;
         int64_t list_off = 0;
         for( int64_t s = 0; s < S; ++s )

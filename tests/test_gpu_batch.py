@@ -63,7 +63,8 @@ def _plain(rings, parts, turns, fma):
 
 @pytest.mark.parametrize("cfg", ["fma=0", "fma=0 queues=1 graph=0", "fma=0 turns_per_launch=2",
                                  "fma=0 block=64 waves_per_cu=16 turns_per_launch=3",
-                                 "fma=0 ppl=2 waves_per_cu=20 turns_per_launch=2"])
+                                 "fma=0 ppl=2 waves_per_cu=20 turns_per_launch=2",
+                                 "fma=0 balance=1", "fma=0 balance=1 waves_per_cu=1 queues=8"])
 def test_batch_matches_oracle_exact(cfg):
     from sixtracklib_amd.trackbatch import TrackBatch
     rings, parts = _workload()
