@@ -12,7 +12,9 @@ build) and to the persistent-chunk form (balance=0) in either build:
 * the same for two particles per lane, and for a lattice with the maps
   that run out of line (beam-beam, space charge, RF multipole);
 * the full-occupancy regime (9,375 tiles for 8,192 resident waves) against
-  balance=0, and a sample of its particles against the oracle.
+  balance=0, and a sample of its particles against the oracle;
+* the issue priority of the waves (wave_prio) changes only timing: with it
+  off the results are the same.
 """
 import numpy as np
 import pytest
@@ -62,14 +64,17 @@ def test_more_tiles_than_waves_mixed_turns_match_oracle(ppl):
     for k in range(2):
         oracle_lib.track_until(ob, eb, 15, index=k)
     want = [particles_dict(particles_in(ob, k)) for k in range(2)]
+    rb = _copy(pb)
     got = _run(eb, pb, "fma=0 waves_per_cu=1 ppl=%d" % ppl, (4, 9, 15))
     ref = _run(eb, qb, "fma=0 waves_per_cu=1 balance=0 ppl=%d" % ppl, (4, 9, 15))
+    nop = _run(eb, rb, "fma=0 waves_per_cu=1 wave_prio=0 ppl=%d" % ppl, (4, 9, 15))
     lost = np.count_nonzero(want[0]["state"] == 0)
     assert 500 < lost < 39000, lost
     assert np.all(want[0]["at_turn"][want[0]["state"] == 1] == 15)
     for k in range(2):
         assert_particles_equal(got[k], want[k], what="balanced, set %d" % k)
         assert_particles_equal(ref[k], want[k], what="chunked, set %d" % k)
+        assert_particles_equal(nop[k], want[k], what="balanced without priority, set %d" % k)
 
 
 def test_more_tiles_than_waves_out_of_line_maps_match_oracle():
