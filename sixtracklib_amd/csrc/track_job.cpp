@@ -545,11 +545,21 @@ struct NS(TrackJobBase)
         if( !plan.has_ebe ) return track_until( until_turn );
         if( !begin_status() ) return end_status( false );
         bool ok = true;
+        // the smallest at_turn of the live particles (one select pass, read
+        // back) sets how many one-turn windows the launcher runs
+        int64_t from_turn = -1;
+        if( compact && n_elem > 0 )
+        {
+            int64_t n_active = 0, min_turn = 0;
+            if( !ensure_runner() || !runner.select_sync( until_turn, stream, n_active, min_turn ) )
+                return end_status( false );
+            from_turn = n_active > 0 ? min_turn : until_turn;
+        }
         for( const st_dev_particles_t& v : views )
         {
             hipError_t const err = fma
-                ? st_launch_track_ebe_fma( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), plan.ebe_cap, &ebe_view, d_status, block, stream )
-                : st_launch_track_ebe_exact( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), plan.ebe_cap, &ebe_view, d_status, block, stream );
+                ? st_launch_track_ebe_fma( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), plan.ebe_cap, &ebe_view, d_status, from_turn, block, stream )
+                : st_launch_track_ebe_exact( &v, d_lat, n_elem, until_turn, &ebe_device_cfg(), plan.ebe_cap, &ebe_view, d_status, from_turn, block, stream );
             ok = ok && hip_ok( err, "launch track_elem_by_elem" );
         }
         return end_status( ok );

@@ -179,6 +179,10 @@ struct Args
     st_ebe_config_t cfg;     // ebe mode
     int64_t ebe_cap;         // ebe store capacity in records
     int64_t ebe_slots;       // store slots per particle (capacity / particles)
+    int64_t turn_limit;      // ebe: turns a lane may advance in this launch (0: to until_turn)
+    int64_t win_base;        // ebe: lane t tracks particle (win_base + t) mod n, t < win_n
+    int64_t win_n;           //      (win_n = 0: particle t)
+    int32_t ebe_lanes;       // ebe: 0 every particle, 1 those not alive (state != 1), 2 alive
     st_dev_particles_t O;    // ebe output set
     int32_t* status_out;
 };
@@ -200,9 +204,19 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
 
     bool active0;
     int32_t status = 0;
+    // a launch may cover only part of the requested turns (element by
+    // element in windows); the lane then stops at a turn boundary exactly as
+    // if until_turn were turn_stop
+    int64_t const turn_stop = ( MODE == MODE_EBE && a.turn_limit > 0 &&
+                                a.until_turn - at_turn > a.turn_limit ) ? at_turn + a.turn_limit
+                                                                        : a.until_turn;
     if( MODE == MODE_EBE )
+    {
         active0 = in_range && n_elem > 0 && a.until_turn > at_turn && at_turn >= 0 &&
                   start >= 0 && pid >= 0;
+        if( a.ebe_lanes == 1 ) active0 = active0 && L.state != 1;
+        else if( a.ebe_lanes == 2 ) active0 = active0 && L.state == 1;
+    }
     else if( MODE == MODE_LINE )
         active0 = in_range && n_elem > 0 && L.state == 1;
     else
@@ -305,7 +319,7 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
         }
         if( MODE == MODE_LINE ) break;
         if( lane_in( act ) ) { ++at_turn; L.s = 0.0; }
-        uint64_t const fin = ballot( at_turn >= a.until_turn ) & act;
+        uint64_t const fin = ballot( at_turn >= turn_stop ) & act;
         if( fin != 0u )
         {
             if( lane_in( fin ) ) store_lane( P, i, L, start, at_turn );
@@ -351,7 +365,9 @@ KN( st_k_track_until_lean )( st_dev_particles_t P, Args a )
 __global__ void __launch_bounds__( 64 ) __attribute__( ( amdgpu_waves_per_eu( 1, 1 ) ) )
 KN( st_k_track_ebe )( st_dev_particles_t P, Args a )
 {
-    track_body< MODE_EBE, true >( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
+    int64_t const t = ( int64_t )blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t const i = a.win_n <= 0 ? t : ( t < a.win_n ? ( a.win_base + t ) % P.n : P.n );
+    track_body< MODE_EBE, true >( P, i, a );
 }
 
 __global__ void __launch_bounds__( 256 ) KN( st_k_track_line )( st_dev_particles_t P, Args a )
@@ -396,42 +412,65 @@ extern "C" hipError_t KN( st_launch_track_until )( const st_dev_particles_t* P,
 extern "C" hipError_t KN( st_launch_track_ebe )( const st_dev_particles_t* P,
     const uint64_t* lat, int64_t n_elem, int64_t until_turn,
     const st_ebe_config_t* cfg, int64_t cap, const st_dev_particles_t* O, int32_t* status,
-    int block, hipStream_t stream )
+    int64_t from_turn, int block, hipStream_t stream )
 {
     if( P->n <= 0 ) return hipSuccess;
-    block = sane_block( block );
+    ( void )block;
     Args a{};
     a.lat = ( cptr_t )lat; a.n_elem = n_elem; a.until_turn = until_turn;
     a.cfg = *cfg; a.ebe_cap = cap; a.O = *O; a.status_out = status;
     a.ebe_slots = cfg->num_particles_to_store > 0 ? cap / cfg->num_particles_to_store : 0;
     // The record stores bound this kernel, and a SIMD holding two of its
     // waves takes twice as long as one holding one: the store rate of a
-    // SIMD does not grow with its waves.  So the set is tracked in rounds
-    // of one wave per SIMD (64 particles x 4 SIMDs x CUs), one launch after
-    // the other, the rest in a last one; one wave per workgroup, so that
-    // the waves spread over all CUs.  (Equal rounds of fewer waves measured
-    // slower.)  Every particle is tracked on its own and its records' slots
-    // follow from its id, element and turn, so the rounds give the results
-    // of one launch.
-    ( void )block;
+    // SIMD does not grow with its waves.  So every launch holds at most one
+    // wave per SIMD (W = 64 particles x 4 SIMDs x CUs), one wave per
+    // workgroup so that the waves spread over all CUs:
+    //   1. the particles not alive on entry, in windows of W: each stores
+    //      its record at its element and stops there, as in one launch;
+    //   2. with from_turn (the smallest at_turn of the live particles) known,
+    //      launches of one turn each over windows of W live particles that
+    //      advance by W modulo the set, ceil(n (until - from) / W) of them:
+    //      every particle gets at least its turns (a particle at until_turn
+    //      is left alone), and every launch but the last is full, where
+    //      rounds of the whole request would end on a part-filled round;
+    //   without from_turn, the live particles in rounds of W for the whole
+    //   request.
+    // Every particle is tracked on its own, turn by turn, and its records'
+    // slots follow from its id, element and turn, so these launches give
+    // the results of one launch.  A particle lost in one launch is not alive
+    // in the next.
     int n_cu = 0, dev = 0;
     if( hipGetDevice( &dev ) != hipSuccess ||
         hipDeviceGetAttribute( &n_cu, hipDeviceAttributeMultiprocessorCount, dev ) != hipSuccess ||
         n_cu <= 0 ) n_cu = 1;
-    int64_t const per_round = 64 * 4 * ( int64_t )n_cu;                  // particles
-    for( int64_t begin = 0; begin < P->n; begin += per_round )
+    int64_t const W = 64 * 4 * ( int64_t )n_cu;
+    auto launch = [ & ]( int64_t base, int64_t n, int lanes, int64_t turn_limit ) -> hipError_t
     {
-        st_dev_particles_t sub = *P;
-        sub.n = std::min< int64_t >( per_round, P->n - begin );
-        sub.q0 += begin; sub.mass0 += begin; sub.beta0 += begin; sub.gamma0 += begin;
-        sub.p0c += begin; sub.s += begin; sub.x += begin; sub.y += begin; sub.px += begin;
-        sub.py += begin; sub.zeta += begin; sub.psigma += begin; sub.delta += begin;
-        sub.rpp += begin; sub.rvv += begin; sub.chi += begin; sub.charge_ratio += begin;
-        sub.particle_id += begin; sub.at_element += begin; sub.at_turn += begin;
-        sub.state += begin;
-        hipLaunchKernelGGL( KN( st_k_track_ebe ), dim3( grid_for( sub.n, 64 ) ), dim3( 64 ), 0,
-                            stream, sub, a );
-        hipError_t const e = hipGetLastError();
+        a.win_base = base; a.win_n = n; a.ebe_lanes = lanes; a.turn_limit = turn_limit;
+        hipLaunchKernelGGL( KN( st_k_track_ebe ), dim3( grid_for( n, 64 ) ), dim3( 64 ), 0, stream,
+                            *P, a );
+        return hipGetLastError();
+    };
+    for( int64_t begin = 0; begin < P->n; begin += W )
+    {
+        hipError_t const e = launch( begin, std::min( W, P->n - begin ), 1, 0 );
+        if( e != hipSuccess ) return e;
+    }
+    if( from_turn < 0 || from_turn >= until_turn )
+    {
+        for( int64_t begin = 0; from_turn < 0 && begin < P->n; begin += W )
+        {
+            hipError_t const e = launch( begin, std::min( W, P->n - begin ), 2, 0 );
+            if( e != hipSuccess ) return e;
+        }
+        return hipSuccess;
+    }
+    int64_t const win = std::min( W, P->n );
+    int64_t const turns = until_turn - from_turn;
+    int64_t const launches = ( P->n * turns + win - 1 ) / win;
+    for( int64_t j = 0; j < launches; ++j )
+    {
+        hipError_t const e = launch( ( j * win ) % P->n, win, 2, 1 );
         if( e != hipSuccess ) return e;
     }
     return hipSuccess;
