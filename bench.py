@@ -45,7 +45,7 @@ HBM_PEAK_GBS = 8000.0
 # and the 1-turn element-by-element launch of the collective ring with
 # 100k particles (740,191 / 211,709,952 KiB)
 TRACK_LAUNCH_HBM_BYTES = 450007 * 2048 + 73859 * 1024
-EBE_TURN_HBM_BYTES = 123975 * 2048 + 205186743 * 1024
+EBE_TURN_HBM_BYTES = 151306 * 2048 + 205193746 * 1024
 
 
 def sigma_at_start(eb):

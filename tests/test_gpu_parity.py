@@ -374,9 +374,10 @@ def test_elem_by_elem_store_layouts_match_oracle(n, turns, shuffle):
 def test_elem_by_elem_in_rounds_matches_oracle():
     """A set larger than one wave per SIMD (70,000 particles; 65,536 fill
     the 1,024 SIMDs of a 256-CU device) is tracked element by element in
-    rounds of one wave per SIMD, one launch after the other; records and
-    particles equal the oracle's bit for bit, losses at the aperture
-    included."""
+    one-turn launches over windows of 65,536 particles that rotate through
+    the set (4 launches for 3 turns, windows wrapping past the end);
+    records and particles equal the oracle's bit for bit, losses at the
+    aperture included."""
     from sixtracklib_amd.trackjob import TrackJob
     from sixtracklib_amd.particles import particles_in, Particles
     eb = CBuffer()
