@@ -388,12 +388,18 @@ def test_elem_by_elem_in_rounds_matches_oracle():
     E, T, n = 4, 3, 70000
     pb, _ = lattices.gaussian_beam(n, seed=8, sigma_x=4e-3, sigma_y=4e-3, sigma_px=2e-4,
                                    sigma_py=2e-4)
-    ob = _copy(pb)
+    ob, cb = _copy(pb), _copy(pb)
     job = TrackJob(eb, pb, until_turn_elem_by_elem=T, config_str="fma=0")
     off = job.elem_by_elem_output_offset
     assert job.track_elem_by_elem(T) == 0
     job.collect()
     got = _pdict(job.output, off)
+    # compact=0: no select pass, the live particles in rounds of one wave
+    # per SIMD for the whole request
+    job2 = TrackJob(eb, cb, until_turn_elem_by_elem=T, config_str="fma=0 compact=0")
+    assert job2.track_elem_by_elem(T) == 0
+    job2.collect()
+    got2 = _pdict(job2.output, job2.elem_by_elem_output_offset)
     ocb = CBuffer(max_slots=22 * n * E * T + 64)
     Particles(cbuffer=ocb, num_particles=n * E * T)
     cfg = oracle_lib.EbeConfig(order=0, num_particles_to_store=n, num_elements_to_store=E,
@@ -406,6 +412,8 @@ def test_elem_by_elem_in_rounds_matches_oracle():
     assert 1000 < np.count_nonzero(want["state"] == 0) < n - 1000
     assert_particles_equal(_pdict(pb), want, what="particles")
     assert_particles_equal(got, _pdict(ocb), what="store")
+    assert_particles_equal(_pdict(cb), want, what="particles, compact=0")
+    assert_particles_equal(got2, _pdict(ocb), what="store, compact=0")
 
 
 @pytest.mark.parametrize("passes", [300, 407 * 4 - 5])
