@@ -428,11 +428,12 @@ extern "C" hipError_t KN( st_launch_track_ebe )( const st_dev_particles_t* P,
     //   1. the particles not alive on entry, in windows of W: each stores
     //      its record at its element and stops there, as in one launch;
     //   2. with from_turn (the smallest at_turn of the live particles) known,
-    //      launches of one turn each over windows of W live particles that
-    //      advance by W modulo the set, ceil(n (until - from) / W) of them:
-    //      every particle gets at least its turns (a particle at until_turn
-    //      is left alone), and every launch but the last is full, where
-    //      rounds of the whole request would end on a part-filled round;
+    //      L = ceil(n (until - from) / W) launches of one turn each over
+    //      windows of w <= W particles (n (until - from) / L rounded up to
+    //      whole waves) that advance by w modulo the set: every particle gets
+    //      at least its turns (L w >= n (until - from); a particle at
+    //      until_turn is left alone), and the launches are evenly filled,
+    //      where rounds of the whole request would end on a part-filled one;
     //   without from_turn, the live particles in rounds of W for the whole
     //   request.
     // Every particle is tracked on its own, turn by turn, and its records'
@@ -465,9 +466,13 @@ extern "C" hipError_t KN( st_launch_track_ebe )( const st_dev_particles_t* P,
         }
         return hipSuccess;
     }
-    int64_t const win = std::min( W, P->n );
+    // the fewest launches of at most W particles, then the window shrunk to
+    // spread the particle-turns evenly over them (whole waves), so that the
+    // last launch is not a few waves at the latency floor
     int64_t const turns = until_turn - from_turn;
-    int64_t const launches = ( P->n * turns + win - 1 ) / win;
+    int64_t const launches = ( P->n * turns + std::min( W, P->n ) - 1 ) / std::min( W, P->n );
+    int64_t const even = ( P->n * turns + launches - 1 ) / launches;
+    int64_t const win = std::min( std::min( W, P->n ), 64 * ( ( even + 63 ) / 64 ) );
     for( int64_t j = 0; j < launches; ++j )
     {
         hipError_t const e = launch( ( j * win ) % P->n, win, 2, 1 );
