@@ -214,6 +214,16 @@ ST_DEV void multipole( Lane& p, double const chi, cptr_t e, unsigned aux,
     double dpx, dpy;
     double const x = p.x, y = p.y;
     horner( e, order, x, y, dpx, dpy );
+#if defined( ST_FMA_BUILD )
+    // px -= chi dpx, py += chi dpy as one fused step each (the common case;
+    // contraction does not reach across the curvature branch by itself)
+    if( !( aux & ST_MP_CURV_FLAG ) )
+    {
+        p.px = fma( -chi, dpx, p.px );
+        p.py = fma( chi, dpy, p.py );
+        return;
+    }
+#endif
     dpx = -chi * dpx;
     dpy = chi * dpy;
     if( aux & ST_MP_CURV_FLAG )
