@@ -359,7 +359,7 @@ KN( st_k_track_until_lean )( st_dev_particles_t P, Args a )
     track_until_lean( P, ( int64_t )blockIdx.x * blockDim.x + threadIdx.x, a );
 }
 
-// One wave per workgroup and, in the launcher's rounds, one per SIMD: the
+// One wave per workgroup and, in the launcher's windows, one per SIMD: the
 // whole register file is this wave's, so the full map set keeps its state
 // and the record fields in registers (no scratch spills on the store path).
 __global__ void __launch_bounds__( 64 ) __attribute__( ( amdgpu_waves_per_eu( 1, 1 ) ) )

@@ -371,7 +371,7 @@ def test_elem_by_elem_store_layouts_match_oracle(n, turns, shuffle):
     assert_particles_equal(got, _pdict(ocb), what="store")
 
 
-def test_elem_by_elem_in_rounds_matches_oracle():
+def test_elem_by_elem_in_windows_matches_oracle():
     """A set larger than one wave per SIMD (70,000 particles; 65,536 fill
     the 1,024 SIMDs of a 256-CU device) is tracked element by element in
     one-turn launches over windows of 65,536 particles that rotate through
