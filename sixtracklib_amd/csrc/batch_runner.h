@@ -15,7 +15,7 @@
 // run_balanced(U) is the alternative for one request on a host thread that
 // may wait: every launch gives each resident wave at most one tile, taken
 // from the particles furthest behind first, and the host reads the counts
-// back between launches.  With T tiles and W resident waves, W < T < 2W,
+// back between groups of launches.  With T tiles and W resident waves, W < T < 2W,
 // the persistent form ends every chunk with T - W tiles on a nearly idle
 // chip; here every launch but the last of a request fills the W waves.
 #pragma once
