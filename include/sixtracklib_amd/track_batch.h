@@ -34,8 +34,8 @@
  *                         between launch groups (no graph; results
  *                         identical)
  * NS(TrackBatch_autotune) picks particles per lane, block, waves_per_cu
- * and queues by timing short trials on the batch itself (state restored
- * afterwards).
+ * and queues by timing short trials on the batch itself, the three fastest
+ * twice (state restored afterwards).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_BATCH_H__
 #define SIXTRACKLIB_AMD_TRACK_BATCH_H__

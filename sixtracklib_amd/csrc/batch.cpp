@@ -18,6 +18,7 @@
 #include <climits>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "sixtracklib_amd/track_batch.h"
@@ -226,9 +227,9 @@ struct NS(TrackBatch)
     }
 
     // Candidate (particles per lane, block, waves_per_cu, queues) timed on
-    // trial_turns turns of the batch itself; the particle images are saved
-    // first and restored after every trial, so the batch state is
-    // unchanged.  One particle per lane is tried at 4 and 8 resident waves
+    // trial_turns turns of the batch itself, the three fastest a second
+    // time; the particle images are saved first and restored after every
+    // trial, so the batch state is unchanged.  One particle per lane is tried at 4 and 8 resident waves
     // per SIMD, two particles per lane at 2.5 and 5 (their register
     // budgets allow 8 and 5).
     int autotune( int64_t trial_turns )
@@ -241,31 +242,50 @@ struct NS(TrackBatch)
                           "backup" ) && hip_ok( hipStreamSynchronize( stream ), "sync" );
         Config const base = run.cfg;
         int64_t const t0 = min_turn;
-        Config best = base;
-        float best_ms = -1.0f;
+        // one trial per candidate, then the three fastest timed once more and
+        // ranked by the better of their two trials: a single short trial
+        // can rank a slower candidate first
+        auto trial = [ & ]( Config const& c, float& ms ) -> bool
+        {
+            run.cfg = c;
+            if( !run.build_queues( stream ) ) return false;
+            min_turn = t0;
+            int32_t const st = run.cfg.balance ? track_until_balanced( t0 + trial_turns )
+                                               : track_until( t0 + trial_turns, trial_turns );
+            ms = last_ms;
+            return st >= -1 && last_ms >= 0.0f &&
+                   hip_ok( hipMemcpyAsync( d_img, backup, total_img, hipMemcpyDeviceToDevice, stream ),
+                           "restore" ) &&
+                   hip_ok( hipStreamSynchronize( stream ), "sync" );
+        };
+        std::vector< std::pair< float, Config > > timed;
         int const blocks[] = { 64, 128, 256 };
         int const waves[ 2 ][ 2 ] = { { 16, 32 }, { 10, 20 } };
         int const queues[] = { 1, 8 };
-        for( int ppl = 1; ppl <= 2; ++ppl )
-        for( int b : blocks )
-            for( int w : waves[ ppl - 1 ] )
-                for( int q : queues )
-                {
-                    if( !ok ) break;
-                    run.cfg = base;
-                    run.cfg.ppl = ppl;
-                    run.cfg.block = b; run.cfg.waves_per_cu = w; run.cfg.queues = q;
-                    run.cfg.graph = false;
-                    if( !run.build_queues( stream ) ) { ok = false; break; }
-                    min_turn = t0;
-                    int32_t const st = run.cfg.balance ? track_until_balanced( t0 + trial_turns )
-                                                       : track_until( t0 + trial_turns, trial_turns );
-                    ok = st >= -1 && last_ms >= 0.0f;
-                    if( ok && ( best_ms < 0.0f || last_ms < best_ms ) ) { best_ms = last_ms; best = run.cfg; }
-                    ok = ok && hip_ok( hipMemcpyAsync( d_img, backup, total_img, hipMemcpyDeviceToDevice,
-                                                       stream ), "restore" ) &&
-                         hip_ok( hipStreamSynchronize( stream ), "sync" );
-                }
+        for( int ppl = 1; ok && ppl <= 2; ++ppl )
+            for( int b : blocks )
+                for( int w : waves[ ppl - 1 ] )
+                    for( int q : queues )
+                    {
+                        if( !ok ) break;
+                        Config c = base;
+                        c.ppl = ppl; c.block = b; c.waves_per_cu = w; c.queues = q;
+                        c.graph = false;
+                        float ms = -1.0f;
+                        ok = trial( c, ms );
+                        if( ok ) timed.emplace_back( ms, c );
+                    }
+        std::stable_sort( timed.begin(), timed.end(),
+                          []( auto const& x, auto const& y ) { return x.first < y.first; } );
+        Config best = base;
+        float best_ms = -1.0f;
+        for( size_t r = 0; ok && r < std::min< size_t >( 3, timed.size() ); ++r )
+        {
+            float ms = -1.0f;
+            ok = trial( timed[ r ].second, ms );
+            float const m = std::min( ms, timed[ r ].first );
+            if( ok && ( best_ms < 0.0f || m < best_ms ) ) { best_ms = m; best = timed[ r ].second; }
+        }
         ( void )hipFree( backup );
         run.cfg = best;
         run.cfg.graph = base.graph;
