@@ -33,9 +33,9 @@
  *                         behind first, the host reading the counts back
  *                         between launch groups (no graph; results
  *                         identical)
- * NS(TrackBatch_autotune) picks particles per lane, block, waves_per_cu
- * and queues by timing short trials on the batch itself, the three fastest
- * twice (state restored afterwards).
+ * NS(TrackBatch_autotune) picks block, waves_per_cu and queues for one
+ * particle per lane by timing short trials on the batch itself, the three
+ * fastest twice (state restored afterwards).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_BATCH_H__
 #define SIXTRACKLIB_AMD_TRACK_BATCH_H__

@@ -226,12 +226,11 @@ struct NS(TrackBatch)
         return *h_status;
     }
 
-    // Candidate (particles per lane, block, waves_per_cu, queues) timed on
-    // trial_turns turns of the batch itself, the three fastest a second
-    // time; the particle images are saved first and restored after every
-    // trial, so the batch state is unchanged.  One particle per lane is tried at 4 and 8 resident waves
-    // per SIMD, two particles per lane at 2.5 and 5 (their register
-    // budgets allow 8 and 5).
+    // Candidate (block, waves_per_cu, queues) timed on trial_turns turns of
+    // the batch itself, the three fastest a second time; the particle images
+    // are saved first and restored after every trial, so the batch state is
+    // unchanged.  One particle per lane, at 4 and 8 resident waves per SIMD
+    // (its register budget allows 8).
     int autotune( int64_t trial_turns )
     {
         if( trial_turns <= 0 ) trial_turns = 2;
@@ -258,18 +257,20 @@ struct NS(TrackBatch)
                            "restore" ) &&
                    hip_ok( hipStreamSynchronize( stream ), "sync" );
         };
+        // One particle per lane only: two per lane (128-particle tiles) won
+        // short trials, which start with every particle alive, but lost
+        // every full run with losses; it stays available as ppl=2.
         std::vector< std::pair< float, Config > > timed;
         int const blocks[] = { 64, 128, 256 };
-        int const waves[ 2 ][ 2 ] = { { 16, 32 }, { 10, 20 } };
+        int const waves[] = { 16, 32 };
         int const queues[] = { 1, 8 };
-        for( int ppl = 1; ok && ppl <= 2; ++ppl )
-            for( int b : blocks )
-                for( int w : waves[ ppl - 1 ] )
-                    for( int q : queues )
+        for( int b : blocks )
+            for( int w : waves )
+                for( int q : queues )
                     {
                         if( !ok ) break;
                         Config c = base;
-                        c.ppl = ppl; c.block = b; c.waves_per_cu = w; c.queues = q;
+                        c.ppl = 1; c.block = b; c.waves_per_cu = w; c.queues = q;
                         c.graph = false;
                         float ms = -1.0f;
                         ok = trial( c, ms );
