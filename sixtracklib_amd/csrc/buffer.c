@@ -7,6 +7,8 @@
  * contiguous allocation laid out as header | slots | objects | dataptrs |
  * garbage, grown by re-layout + relocation.
  */
+#define _POSIX_C_SOURCE 200112L     /* posix_memalign under -std=c99 */
+
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
