@@ -19,7 +19,8 @@ events on the job's stream; cpu_baseline = the C99 oracle with OpenMP on
 the host cores over a bounded sample of the same workload.
 
 Beside the headline figure the line carries the other forms of the path:
-"batched" (64 machine seeds in one persistent launch vs 64 plain jobs),
+"batched" (64 machine seeds in one graph of persistent launches vs 64
+plain jobs),
 "adjoint" (backward / forward cost of the differentiable tracker) and
 "elem_by_elem" (element-by-element storage through the ring with
 beam-beam and space-charge elements; an HBM-bound kernel, with its own
@@ -41,9 +42,10 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (KiB per dispatch; a 1 GiB device-to-device copy reads 524,308 KiB of
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
-# the 23-turn track launch of the 1M-particle ring (578,319 / 128,460 KiB)
-# and the 1-turn element-by-element launch of the collective ring with
-# 100k particles (740,191 / 211,709,952 KiB)
+# a full 25-turn track launch of the 1M-particle ring (8,192 tiles; mean of
+# the seven full launches of turns 0-100: 450,007 / 73,859 KiB) and one turn
+# of element-by-element tracking of the collective ring with 100k particles
+# (all windows of two 2-turn calls / 4: 151,306 / 205,193,746 KiB)
 TRACK_LAUNCH_HBM_BYTES = 450007 * 2048 + 73859 * 1024
 EBE_TURN_HBM_BYTES = 151306 * 2048 + 205193746 * 1024
 
