@@ -46,7 +46,10 @@ def _jobs():
                                           os.path.join(CSRC, src), "-o",
                                           os.path.join(BUILD, src + ".o")],
                   os.path.join(BUILD, src + ".o")))
-    for variant, contract in (("_exact", "off"), ("_fma", "fast")):
+    # "fast-honor-pragmas": contraction everywhere except in the blocks that
+    # switch it off (the energy update and the drift's 1 + t/2), where the
+    # reference's own rounding is kept
+    for variant, contract in (("_exact", "off"), ("_fma", "fast-honor-pragmas")):
         for src in ("track_kernels.hip", "batch_kernels.hip", "adjoint_kernels.hip"):
             if not os.path.exists(os.path.join(CSRC, src)):
                 continue

@@ -54,16 +54,24 @@ struct LaneConst
     double q0, mass0, beta0, p0c, chi, charge_ratio;
 };
 
-// be_drift/track.h:45-67.  The FMA build evaluates
-// rvv - (1 + (xp^2 + yp^2)/2) as (rvv - 1) - (xp^2 + yp^2)/2 with fused
-// multiply-adds (same quantity, different rounding).
+// be_drift/track.h:45-67.  dzeta = rvv - (1 + (xp^2 + yp^2)/2) cancels:
+// 1 + t/2 rounds t/2 (about 1e-10) to the spacing of doubles near 1, so
+// the zeta gained per drift carries a relative rounding error of order
+// 1e-7.  The FMA build keeps exactly those roundings (t from two unfused
+// products and a sum, 1 + t/2 rounded once, as fma(t, 0.5, 1) does since
+// t * 0.5 is exact) and fuses only the updates of x, y and zeta, which
+// move each result by at most an ulp of its own size.
 ST_DEV void drift( Lane& p, double const length )
 {
     double const xp = p.px * p.rpp;
     double const yp = p.py * p.rpp;
 #if defined( ST_FMA_BUILD )
-    double const t = fma( xp, xp, yp * yp );
-    double const dzeta = fma( t, -0.5, p.rvv - 1.0 );
+    double t;
+    {
+#pragma clang fp contract( off )
+        t = xp * xp + yp * yp;
+    }
+    double const dzeta = p.rvv - fma( t, 0.5, 1.0 );
     p.x = fma( xp, length, p.x );
     p.y = fma( yp, length, p.y );
     p.s += length;
@@ -77,25 +85,10 @@ ST_DEV void drift( Lane& p, double const length )
 #endif
 }
 
-#if defined( ST_FMA_BUILD )
-// The same map with rvv - 1 supplied by the caller (it changes only where
-// the energy does).
-ST_DEV void drift_rvvm1( Lane& p, double const length, double const rvvm1 )
-{
-    double const xp = p.px * p.rpp;
-    double const yp = p.py * p.rpp;
-    double const t = fma( xp, xp, yp * yp );
-    double const dzeta = fma( t, -0.5, rvvm1 );
-    p.x = fma( xp, length, p.x );
-    p.y = fma( yp, length, p.y );
-    p.s += length;
-    p.zeta = fma( length, dzeta, p.zeta );
-}
-#endif
-
 // be_drift/track.h:71-97
 ST_DEV void drift_exact( Lane& p, double const length )
 {
+#pragma clang fp contract( off )   // rvv * length - opd * lpzi cancels
     double const opd = p.delta + 1.0;
     double const lpzi = length / sqrt( opd * opd - ( p.px * p.px + p.py * p.py ) );
     double const dzeta = p.rvv * length - opd * lpzi;
@@ -256,6 +249,7 @@ ST_DEV void multipole( Lane& p, const LaneConst& c, cptr_t e, unsigned aux,
 // particles.h:5255-5297 (energy0 5220-5234, zeta scaling 5024-5032)
 ST_DEV void add_to_energy( Lane& p, const LaneConst& c, double const dE )
 {
+#pragma clang fp contract( off )
     double const beta0 = c.beta0;
     double const delta_beta0 = p.delta * beta0;
     double const energy0 = sqrt( c.p0c * c.p0c + c.mass0 * c.mass0 );
@@ -276,6 +270,7 @@ ST_DEV void add_to_energy( Lane& p, const LaneConst& c, double const dE )
 // particles.h:5459-5497
 ST_DEV void update_delta( Lane& p, const LaneConst& c, double const new_delta )
 {
+#pragma clang fp contract( off )
     double const beta0 = c.beta0;
     double const delta_beta0 = new_delta * beta0;
     double const ptau_beta0 = sqrt( delta_beta0 * delta_beta0 +
@@ -290,6 +285,7 @@ ST_DEV void update_delta( Lane& p, const LaneConst& c, double const new_delta )
 // be_cavity/track.h:39-67 with DEG2RAD*lag and K_FACTOR*f folded at pack time
 ST_DEV void cavity( Lane& p, const LaneConst& c, cptr_t e )
 {
+#pragma clang fp contract( off )
     double const voltage = rd( e, 0 ), lag_rad = rd( e, 1 ), kfreq = rd( e, 2 );
     double const tau = p.zeta / ( c.beta0 * p.rvv );
     double const phase = lag_rad - kfreq * tau;

@@ -213,9 +213,6 @@ ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
         uint64_t h = rec_at( lat, 0 )[ 0 ];
         for( ;; )
         {
-#if defined( ST_FMA_BUILD )
-            double const rvvm1 = L.rvv - 1.0;         // changes only at a Cavity
-#endif
             while( ( ( uint32_t )h & ST_OP_RARE_BIT ) == 0u )   // Drift / Multipole run
             {
                 cptr_t const r = rec_at( lat, off );
@@ -227,11 +224,7 @@ ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
                 }
                 else
                 {
-#if defined( ST_FMA_BUILD )
-                    drift_rvvm1( L, rd( r, ST_REC_PAYLOAD ), rvvm1 );
-#else
                     drift( L, rd( r, ST_REC_PAYLOAD ) );
-#endif
                     uint64_t const lost = limit_global_lost_mask( L ) & act;
                     if( lost != 0u )
                     {
@@ -326,11 +319,6 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
         uint64_t h = rec_at( lat, 0 )[ 0 ];
         for( ;; )
         {
-#if defined( ST_FMA_BUILD )
-            double rvvm1[ K ];
-#pragma unroll
-            for( int k = 0; k < K; ++k ) rvvm1[ k ] = L[ k ].rvv - 1.0;
-#endif
             while( ( ( uint32_t )h & ST_OP_RARE_BIT ) == 0u )   // Drift / Multipole run
             {
                 cptr_t const r = rec_at( lat, off );
@@ -349,11 +337,7 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
 #pragma unroll
                     for( int k = 0; k < K; ++k )
                     {
-#if defined( ST_FMA_BUILD )
-                        drift_rvvm1( L[ k ], length, rvvm1[ k ] );
-#else
                         drift( L[ k ], length );
-#endif
                     }
 #pragma unroll
                     for( int k = 0; k < K; ++k )
