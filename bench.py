@@ -43,10 +43,10 @@ HBM_PEAK_GBS = 8000.0
 # (KiB per dispatch; a 1 GiB device-to-device copy reads 524,308 KiB of
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
 # a full 25-turn track launch of the 1M-particle ring (8,192 tiles; mean of
-# the seven full launches of turns 0-100: 450,007 / 73,859 KiB) and one turn
+# the seven full launches of turns 0-100: 472,860 / 73,823 KiB) and one turn
 # of element-by-element tracking of the collective ring with 100k particles
 # (all windows of two 2-turn calls / 4: 151,306 / 205,193,746 KiB)
-TRACK_LAUNCH_HBM_BYTES = 450007 * 2048 + 73859 * 1024
+TRACK_LAUNCH_HBM_BYTES = 472860 * 2048 + 73823 * 1024
 EBE_TURN_HBM_BYTES = 151306 * 2048 + 205193746 * 1024
 
 
