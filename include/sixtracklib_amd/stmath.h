@@ -39,6 +39,9 @@
 
 #include <math.h>
 
+/* Every function below switches contraction off (clang only; the C build
+ * of the oracle compiles with -ffp-contract=off), so that a device build
+ * made with contraction still produces the same bits as the host. */
 #if defined( __HIPCC__ )
     #define ST_MATH_FN __host__ __device__ static inline
 #else
@@ -60,6 +63,9 @@
 /* sin(r) on |r| <= pi/4 : r + r^3 * (S1 + r^2 * (S2 + ...)) */
 ST_MATH_FN double st_sin_kernel( double const r )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     double const z = r * r;
     double p = -0x1.2f49b46814157p-57;      /* -1/19! */
     p = p * z + 0x1.952c77030ad4ap-49;      /* +1/17! */
@@ -77,6 +83,9 @@ ST_MATH_FN double st_sin_kernel( double const r )
  * with the 1 - r^2/2 step compensated for its rounding error. */
 ST_MATH_FN double st_cos_kernel( double const r )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     double const z = r * r;
     double q = 0x1.e542ba4020225p-62;       /* +1/20! */
     q = q * z - 0x1.6827863b97d97p-53;      /* -1/18! */
@@ -98,6 +107,9 @@ ST_MATH_FN double st_cos_kernel( double const r )
  * occur. */
 ST_MATH_FN double st_reduce_pio2( double const x, int* quadrant )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     double const k = rint( x * ST_TWO_OVER_PI );
     double r = x - k * ST_PIO2_1;
     r = r - k * ST_PIO2_2;
@@ -112,6 +124,9 @@ ST_MATH_FN double st_reduce_pio2( double const x, int* quadrant )
  * Outside the domain (|x| >= 2^30, Inf, NaN) the result is NaN. */
 ST_MATH_FN double st_sin( double const x )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     int const finite = fabs( x ) < 0x1p+30;   /* domain; false for Inf / NaN */
     int q = 0;
     double const r = st_reduce_pio2( finite ? x : 0.0, &q );
@@ -123,6 +138,9 @@ ST_MATH_FN double st_sin( double const x )
 
 ST_MATH_FN double st_cos( double const x )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     int const finite = fabs( x ) < 0x1p+30;
     int q = 0;
     double const r = st_reduce_pio2( finite ? x : 0.0, &q );
@@ -134,6 +152,9 @@ ST_MATH_FN double st_cos( double const x )
 
 ST_MATH_FN double st_exp( double const x )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     if( x != x ) return x;
     if( x >  709.782712893384 ) return ( double )INFINITY;
     if( x < -745.1332191019412 ) return 0.0;
@@ -158,6 +179,9 @@ ST_MATH_FN double st_exp( double const x )
 
 ST_MATH_FN double st_log( double const x )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     if( x != x ) return x;
     if( x < 0.0 ) return ( x - x ) / 0.0;                  /* NaN */
     if( x == 0.0 ) return -( double )INFINITY;
@@ -189,6 +213,9 @@ ST_MATH_FN double st_log( double const x )
 /* x^n for integer n by binary exponentiation. */
 ST_MATH_FN double st_pow_int( double const x, int const n )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     unsigned int m = ( n < 0 ) ? ( unsigned int )( -n ) : ( unsigned int )n;
     double base = x;
     double acc = 1.0;
@@ -205,6 +232,9 @@ ST_MATH_FN double st_pow_int( double const x, int const n )
  * profiles); negative x gives NaN unless y is integral. */
 ST_MATH_FN double st_pow( double const x, double const y )
 {
+#if defined( __clang__ )
+#pragma clang fp contract( off )
+#endif
     if( y == 0.0 ) return 1.0;
     if( ( y == rint( y ) ) && ( fabs( y ) < 1024.0 ) )
         return st_pow_int( x, ( int )y );

@@ -352,10 +352,16 @@ ST_DEV void dipedge( Lane& p, cptr_t e )
     p.py += rd( e, 1 ) * p.y;
 }
 
+// From here to the space-charge maps every function switches contraction
+// off, also in the FMA build: the field formulas cancel (1 - exp(-r^2/2s^2)
+// at small r, differences of the beam-size terms), so a fused multiply-add
+// there changes a kick by far more than an ulp of the reference's.  These
+// maps run a few times per turn and cost nothing measurable.
 // be_rfmultipole/track.h:39-118
 ST_DEV_NOINLINE void rf_multipole( Lane& p, const LaneConst& c, cptr_t e,
                                    unsigned aux )
 {
+#pragma clang fp contract( off )
     int const order = ( int )aux;
     int const nb = 2 * order + 2;
     double const voltage = rd( e, 0 ), frequency = rd( e, 1 ), lag = rd( e, 2 );
@@ -403,6 +409,7 @@ ST_DEV_NOINLINE void rf_multipole( Lane& p, const LaneConst& c, cptr_t e,
 ST_DEV void cerrf( double const in_real, double const in_imag, double& out_re,
                    double& out_im )
 {
+#pragma clang fp contract( off )
     double const a_constant = 1.12837916709551;
     double const xLim = 5.33;
     double const yLim = 4.29;
@@ -470,6 +477,7 @@ ST_DEV void cerrf( double const in_real, double const in_imag, double& out_re,
 ST_DEV void field_round( double const sigma, double const x, double const y,
                          double& Ex, double& Ey )
 {
+#pragma clang fp contract( off )
     double const Delta_x = 0., Delta_y = 0.;
     double const r2 = ( x - Delta_x ) * ( x - Delta_x ) + ( y - Delta_y ) * ( y - Delta_y );
     double temp;
@@ -485,6 +493,7 @@ ST_DEV void field_ellip( double const sigmax, double const sigmay,
                          double const x, double const y, double& Ex_out,
                          double& Ey_out )
 {
+#pragma clang fp contract( off )
     double const abx = fabs( x - 0. );
     double const aby = fabs( y - 0. );
     double Ex, Ey;
@@ -532,6 +541,7 @@ ST_DEV void gauss_field( double const x, double const y, double const sigma_x,
                          double const sigma_y, double const min_sigma_diff,
                          double& Ex, double& Ey, double& Gx, double& Gy )
 {
+#pragma clang fp contract( off )
     if( fabs( sigma_x - sigma_y ) < min_sigma_diff )
     {
         double const sigma = 0.5 * ( sigma_x + sigma_y );
@@ -569,6 +579,7 @@ ST_DEV void gauss_field( double const x, double const y, double const sigma_x,
 // sigma_y, beta_s, min_sigma_diff, Delta_x, Delta_y, Dpx_sub, Dpy_sub, enabled)
 ST_DEV_NOINLINE void bb4d( Lane& p, const LaneConst& c, cptr_t d )
 {
+#pragma clang fp contract( off )
     if( !( rd( d, 10 ) > 0. ) ) return;
     double const charge = 1. * c.q0 * QELEM;
     double const x = p.x - rd( d, 6 );
@@ -592,6 +603,7 @@ ST_DEV void bb6d_sigma( cptr_t d, double const S, double const threshold_singula
     double& dS_Sig_11_hat, double& dS_Sig_33_hat, double& dS_costheta,
     double& dS_sintheta )
 {
+#pragma clang fp contract( off )
     double const Sig_11_0 = rd( d, 6 ), Sig_12_0 = rd( d, 7 ), Sig_13_0 = rd( d, 8 );
     double const Sig_14_0 = rd( d, 9 ), Sig_22_0 = rd( d, 10 ), Sig_23_0 = rd( d, 11 );
     double const Sig_24_0 = rd( d, 12 ), Sig_33_0 = rd( d, 13 ), Sig_34_0 = rd( d, 14 );
@@ -682,6 +694,7 @@ ST_DEV void bb6d_sigma( cptr_t d, double const S, double const threshold_singula
 // 27-32 D*_sub, 33 enabled, 38.. slices (N, x*, y*, sigma*).
 ST_DEV_NOINLINE void bb6d( Lane& p, const LaneConst& c, cptr_t d )
 {
+#pragma clang fp contract( off )
     if( !( rd( d, 33 ) > 0. ) ) return;
     int const N_slices = ( int )ri( d, 18 );
     cptr_t N_part_per_slice = d + 38;
@@ -806,6 +819,7 @@ ST_DEV_NOINLINE void bb6d( Lane& p, const LaneConst& c, cptr_t d )
 // sigma_y, length, x_co, y_co, min_sigma_diff, enabled)
 ST_DEV_NOINLINE void sc_coasting( Lane& p, const LaneConst& c, cptr_t e )
 {
+#pragma clang fp contract( off )
     double const charge = CHARGE0_SI * c.q0;
     double const beta0 = c.beta0;
     double const pc0 = c.p0c * CHARGE0_SI;
@@ -824,6 +838,7 @@ ST_DEV_NOINLINE void sc_coasting( Lane& p, const LaneConst& c, cptr_t e )
 // min_sigma_diff, q_param, cq, enabled)
 ST_DEV_NOINLINE void sc_qgauss( Lane& p, const LaneConst& c, cptr_t e )
 {
+#pragma clang fp contract( off )
     double const charge = CHARGE0_SI * c.q0;
     double const beta0 = c.beta0;
     double const beta = beta0 * p.rvv;
@@ -863,6 +878,7 @@ ST_DEV_NOINLINE void sc_qgauss( Lane& p, const LaneConst& c, cptr_t e )
 // (where the reference reads one element past the table) yields 0.
 ST_DEV_NOINLINE void sc_interpol( Lane& p, const LaneConst& c, cptr_t e )
 {
+#pragma clang fp contract( off )
     double const charge = CHARGE0_SI * c.q0;
     double const beta0 = c.beta0;
     double const beta = beta0 * p.rvv;
