@@ -5,12 +5,14 @@ sixtracklib_amd.lattices.lhc_like_ring(seed=0) and 1,048,576 particles on a
 polar grid (16,384 amplitudes in (0, 20] sigma x 64 angles, delta 2.7e-4),
 tracked 1000 turns with the HIP tracker (TrackJob, FMA build by default).
 
-A step is one track_until call advancing the whole resident particle set by
-``--turns-per-step`` turns (default 100, so the default 10 steps are the
-1000-turn job).  Warmup steps run first on the same particles; the initial
-state is then pushed again so the timed steps cover turns 0..steps*tps.
-Only tracked particle-turns count: sum over particles of (final at_turn -
-initial at_turn).
+The timed region is always the whole 1000-turn job (``--turns``): a step
+is one track_until call, and the K steps split the turns evenly (step k
+tracks the resident particle set to turn round(1000 (k + 1) / K)), so 10
+steps are 100 turns each and 20 steps 50 each.  Warmup steps run first on
+the same particles, the first W steps of the same split; the initial state
+is then pushed again so the timed steps cover turns 0..1000.  Only tracked
+particle-turns count: sum over particles of (final at_turn - initial
+at_turn).
 
 Prints one JSON line (see the "roofline" and "cpu_baseline" fields):
 roofline achieved = algorithmic FP64 flops of the tracked particle-turns
@@ -82,11 +84,7 @@ def cpu_baseline(eb, pb, budget_s=10.0, sample=8192, turns_per_chunk=2):
     idx = np.linspace(0, n - 1, min(sample, n)).astype(np.int64)
     sb, sp = lattices.make_particles(len(idx), x=np.array(p.x)[idx], y=np.array(p.y)[idx],
                                      delta=2.7e-4)
-    try:
-        threads = len(os.sched_getaffinity(0))
-    except AttributeError:
-        threads = os.cpu_count() or 1
-    threads = max(1, min(threads, 16))
+    threads = cpu_threads()
     at0 = np.array(sp.at_turn).copy()
     t0 = time.perf_counter()
     turn = 0
@@ -99,6 +97,42 @@ def cpu_baseline(eb, pb, budget_s=10.0, sample=8192, turns_per_chunk=2):
     tracked = int(np.sum(np.array(sp.at_turn) - at0))
     return tracked / dt, threads, ("oracle (C99 restatement, OpenMP) on %d of the %d particles "
                                    "x %d turns of the same ring, %.1f s" % (len(idx), n, turn, dt))
+
+
+def host_cores():
+    """(logical CPUs of the host, physical cores of the host, CPUs this
+    process may run on)."""
+    logical = os.cpu_count() or 1
+    phys = set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            pid = cid = None
+            for line in f:
+                if line.startswith("physical id"):
+                    pid = line.split(":", 1)[1].strip()
+                elif line.startswith("core id"):
+                    cid = line.split(":", 1)[1].strip()
+                elif not line.strip():
+                    if cid is not None:
+                        phys.add((pid, cid))
+                    pid = cid = None
+    except OSError:
+        pass
+    try:
+        allowed = len(os.sched_getaffinity(0))
+    except AttributeError:
+        allowed = logical
+    return logical, (len(phys) or logical), allowed
+
+
+def cpu_threads():
+    """Threads for the CPU baseline: every CPU the process may use, up to
+    OMP_NUM_THREADS when that is set (the host share given to one GPU)."""
+    _, _, allowed = host_cores()
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        allowed = min(allowed, int(env))
+    return max(1, allowed)
 
 
 def batched_vs_sequential(n_seeds, turns):
@@ -121,8 +155,11 @@ def batched_vs_sequential(n_seeds, turns):
 
     jobs = [TrackJob(eb, pb, config_str="fma=1") for eb, pb in zip(rings, parts)]
     t0 = time.perf_counter()
-    for j in jobs:
+    for k, j in enumerate(jobs):
         j.track_until(turns)
+        if (k + 1) % 8 == 0:
+            progress("sequential jobs: %d of %d done, %.1f s" % (k + 1, len(jobs),
+                                                                 time.perf_counter() - t0))
     seq_wall = time.perf_counter() - t0
     progress("sequential jobs: %.1f s" % seq_wall)
     for j in jobs:
@@ -190,7 +227,7 @@ def adjoint_cost(reps=3):
     return out
 
 
-def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
+def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2, fma=False):
     """Element-by-element form: lattices.collective_ring() (20,019
     elements: the 20k ring + 2 BeamBeam6D x 5 slices + 1 BeamBeam4D + 16
     SCQGaussProfile), a 6D Gaussian beam matched at the ring start, every
@@ -199,7 +236,8 @@ def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
     (ebe_store_passes; 100k x 20,019 x 10 records would be 3.4 TB).
     Achieved HBM rate = 168 B x records written / kernel time (HIP events
     on the job's stream); best of ``reps`` runs from the same initial
-    state."""
+    state.  The exact build (fma=0) by default: bit-identical to the
+    oracle, and the kernel is bound by its stores, not its arithmetic."""
     from sixtracklib_amd import lattices
     from sixtracklib_amd.particles import particles_in
     from sixtracklib_amd.trackjob import TrackJob
@@ -207,7 +245,7 @@ def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
     pb, p = lattices.matched_gaussian_beam(n_particles)
     init = pb.words.copy()
     job = TrackJob(eb, pb, until_turn_elem_by_elem=turns,
-                   config_str="fma=1 ebe_store_passes=%d" % passes)
+                   config_str="fma=%d ebe_store_passes=%d" % (int(fma), passes))
     n_elem = eb.n_objects
     best = None
     for _ in range(reps):
@@ -266,6 +304,7 @@ def elem_by_elem_collective(n_particles=100000, turns=10, passes=256, reps=2):
                         "BeamBeam6D x 5 slices + BeamBeam4D + 16 SCQGaussProfile); %d "
                         "particles, matched 6D Gaussian; %d turns; rolling store of %d "
                         "passes per particle" % (n_elem, n_particles, turns, passes),
+            "build": "fma" if fma else "exact",
             "particles": n_particles, "turns": turns, "elements": n_elem,
             "store_passes": passes, "store_gb": n_particles * passes * 168 / 1e9,
             "records": records, "records_per_second": records / (kms / 1e3),
@@ -318,7 +357,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--turns-per-step", type=int, default=100)
+    ap.add_argument("--turns", type=int, default=1000,
+                    help="turns of the timed job, split evenly over the steps")
     ap.add_argument("--particles", type=int, default=1 << 20)
     ap.add_argument("--mode", choices=("fma", "exact"), default="fma")
     ap.add_argument("--block", type=int, default=256)
@@ -328,7 +368,7 @@ def main():
                     help="extra timed steps with the exact (no-FMA) build, reported beside")
     ap.add_argument("--batch-seeds", type=int, default=64,
                     help="batched form: machine seeds (0 disables the batched measurement)")
-    ap.add_argument("--batch-turns", type=int, default=100,
+    ap.add_argument("--batch-turns", type=int, default=1000,
                     help="batched form: turns tracked by the batch and by the sequential jobs")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the differentiable-tracking cost measurement")
@@ -355,13 +395,15 @@ def main():
     p = particles_in(pb)
     init_bytes = pb.words.copy()
     flops_pt, n_trans = lattices.flops_per_particle_turn(eb)
-    tps = args.turns_per_step
+    steps = max(1, args.steps)
+    # step k ends at turn targets[k]; the steps together are the whole job
+    targets = [int(round(args.turns * (k + 1) / steps)) for k in range(steps)]
 
     def run(mode, steps, warmup):
         pb.words[:] = init_bytes
         job = TrackJob(eb, pb, config_str="fma=%d block=%d" % (mode == "fma", args.block))
         for w in range(warmup):
-            job.track_until((w + 1) * tps)
+            job.track_until(targets[min(w, len(targets) - 1)])
         if warmup:
             pb.words[:] = init_bytes
             job.push_particles()
@@ -369,7 +411,7 @@ def main():
         kernel_ms = 0.0
         t0 = time.perf_counter()
         for k in range(steps):
-            job.track_until((k + 1) * tps)
+            job.track_until(targets[k])
             kernel_ms += job.last_kernel_time_ms
         t1 = time.perf_counter()
         job.collect_particles()
@@ -381,7 +423,7 @@ def main():
     setup_s = time.perf_counter() - t_setup
     if args.calibrate_copy_mib > 0:
         calibration_copy(args.calibrate_copy_mib << 20)
-    tracked, wall, kernel_ms, alive = run(args.mode, args.steps, args.warmup)
+    tracked, wall, kernel_ms, alive = run(args.mode, steps, args.warmup)
     progress("plain: %d particle-turns in %.2f s" % (tracked, wall))
     value = tracked / wall
     kernel_s = kernel_ms / 1e3
@@ -397,7 +439,7 @@ def main():
     extra = {}
     if args.exact_steps > 0:
         tr_e, wall_e, kms_e, _ = run("exact" if args.mode == "fma" else "fma",
-                                     args.exact_steps, 0)
+                                     min(args.exact_steps, steps), 0)
         extra = {"other_build": "exact" if args.mode == "fma" else "fma",
                  "other_build_value": tr_e / wall_e,
                  "other_build_tflops": tr_e * flops_pt / (kms_e / 1e3) / 1e12}
@@ -416,20 +458,33 @@ def main():
     if args.ebe_turns > 0:
         ebe = elem_by_elem_collective(args.ebe_particles, args.ebe_turns)
         progress("element by element: %.0f GB/s" % ebe["roofline"]["achieved"])
+        # the FMA build of the same leg, reported beside (its results are
+        # within the FMA tolerance, not bit-identical)
+        f = elem_by_elem_collective(args.ebe_particles, args.ebe_turns, fma=True)
+        ebe["fma_build"] = {"records_per_second": f["records_per_second"],
+                            "achieved_gbs": f["roofline"]["achieved"],
+                            "track_until_value": f["track_until_value"]}
+        progress("element by element, FMA build: %.0f GB/s" % f["roofline"]["achieved"])
 
     cpu = None
     if not args.no_cpu_baseline:
         v, threads, desc = cpu_baseline(eb, pb, budget_s=args.cpu_budget)
+        logical, physical, allowed = host_cores()
         cpu = {"value": v, "unit": "particle-turns/s", "cores": threads, "kind": "port",
-               "sample": desc, "cpu_model": cpu_model()}
+               "sample": desc, "cpu_model": cpu_model(), "host_logical_cpus": logical,
+               "host_physical_cores": physical, "cpus_available_to_process": allowed,
+               "per_core_value": v / threads,
+               "threads_rule": "all CPUs the process may use, capped by OMP_NUM_THREADS "
+                               "(the host share of one GPU)"}
+        progress("cpu baseline: %.3g particle-turns/s on %d threads" % (v, threads))
 
-    ms_per_step = wall * 1e3 / max(1, args.steps)
+    ms_per_step = wall * 1e3 / steps
     out = {
         "metric": "FP64 particle-turns/s, 20k-elem ring, 1M particles; batched & adjoint vs plain",
         "value": value,
         "unit": "particle-turns/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": ms_per_step,
         "higher_is_better": True,
@@ -441,8 +496,9 @@ def main():
                                "(10800 Drift, 7992 Multipole order 0-10, 400 XYShift, "
                                "400 SRotation, 400 LimitEllipse, 8 Cavity); %d particles "
                                "(polar grid 0-20 sigma x 64 angles); %d turns"
-                               % (args.particles, args.steps * tps),
-                   "build": args.mode, "turns_per_step": tps, "block": args.block,
+                               % (args.particles, args.turns),
+                   "build": args.mode, "turns": args.turns,
+                   "turns_per_step": args.turns / steps, "block": args.block,
                    "particles": args.particles, "elements": eb.n_objects,
                    "flops_per_particle_turn": flops_pt,
                    "tracked_particle_turns": tracked, "alive_at_end": alive,

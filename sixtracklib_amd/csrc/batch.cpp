@@ -189,6 +189,7 @@ struct NS(TrackBatch)
     // -- tracking ------------------------------------------------------------
     int32_t track_until( int64_t until_turn, int64_t k_override = 0 )
     {
+        last_ms = -1.0f;
         int64_t const k = k_override > 0 ? k_override : run.chunk_turns( run.total_particles );
         int64_t const chunks = until_turn > min_turn ? ( until_turn - min_turn + k - 1 ) / k : 0;
         *h_status = 0;
@@ -211,6 +212,7 @@ struct NS(TrackBatch)
     // between launch groups (no graph).
     int32_t track_until_balanced( int64_t until_turn )
     {
+        last_ms = -1.0f;
         *h_status = 0;
         if( !hip_ok( hipMemcpyAsync( d_status, h_status, sizeof( int32_t ), hipMemcpyHostToDevice,
                                      stream ), "status reset" ) ||
@@ -252,10 +254,12 @@ struct NS(TrackBatch)
             int32_t const st = run.cfg.balance ? track_until_balanced( t0 + trial_turns )
                                                : track_until( t0 + trial_turns, trial_turns );
             ms = last_ms;
-            return st >= -1 && last_ms >= 0.0f &&
-                   hip_ok( hipMemcpyAsync( d_img, backup, total_img, hipMemcpyDeviceToDevice, stream ),
-                           "restore" ) &&
-                   hip_ok( hipStreamSynchronize( stream ), "sync" );
+            // the images go back to the saved state whatever the trial gave
+            bool const restored =
+                hip_ok( hipMemcpyAsync( d_img, backup, total_img, hipMemcpyDeviceToDevice, stream ),
+                        "restore" ) &&
+                hip_ok( hipStreamSynchronize( stream ), "sync" );
+            return restored && st >= -1 && ms >= 0.0f;
         };
         // One particle per lane only: two per lane (128-particle tiles) won
         // short trials, which start with every particle alive, but lost

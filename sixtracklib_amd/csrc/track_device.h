@@ -313,7 +313,10 @@ ST_DEV void srotation( Lane& p, cptr_t e )
 }
 
 // Aperture predicates (the caller applies state &= ok,
-// particles.h:6493-6505).
+// particles.h:6493-6505).  The two ellipse tests switch contraction off in
+// the FMA build as well: a fused x^2 b^2 + y^2 a^2 rounds differently from
+// the reference's products and sum, and a particle on the boundary could
+// then be kept where the reference loses it (or lost where it is kept).
 // be_limit/track.h:90-111
 ST_DEV bool limit_rect_ok( const Lane& p, cptr_t e )
 {
@@ -324,6 +327,7 @@ ST_DEV bool limit_rect_ok( const Lane& p, cptr_t e )
 // be_limit/track.h:113-137
 ST_DEV bool limit_ellipse_ok( const Lane& p, cptr_t e )
 {
+#pragma clang fp contract( off )
     double temp = p.x;
     double y_squ = p.y;
     temp *= temp;
@@ -337,6 +341,7 @@ ST_DEV bool limit_ellipse_ok( const Lane& p, cptr_t e )
 // be_limit/track.h:139-162
 ST_DEV bool limit_rect_ellipse_ok( const Lane& p, cptr_t e )
 {
+#pragma clang fp contract( off )
     double const x = p.x, y = p.y;
     double const max_x = rd( e, 0 ), max_y = rd( e, 1 );
     double temp = x * x * rd( e, 3 );

@@ -204,11 +204,14 @@ class TrackJob:
 
     # -- tracking ----------------------------------------------------------
     def _check(self, status, what):
+        """python/sixtracklib/trackjob.py:1522-1553: the status is kept in
+        last_track_status, a failure raises, and the job is returned so
+        that calls chain (job.track_until(n).collect())."""
         self._last_track_status = status
         self._last_status = status
         if status != 0:
             raise RuntimeError("%s failed with status %d %s" % (what, status, st.last_error()))
-        return status
+        return self
 
     def track(self, until_turn):
         return self.track_until(until_turn)

@@ -51,7 +51,7 @@ def _run(eb, pb, cfg, steps):
     from sixtracklib_amd.trackjob import TrackJob
     job = TrackJob(eb, pb, config_str=cfg, particle_set_indices=[0, 1])
     for t in steps:
-        assert job.track_until(t) == 0
+        assert job.track_until(t).last_track_status == 0
     job.collect()
     return [particles_dict(particles_in(pb, k)) for k in range(2)]
 
@@ -119,6 +119,6 @@ def _run1(eb, pb, cfg):
     from sixtracklib_amd.trackjob import TrackJob
     job = TrackJob(eb, pb, config_str=cfg)
     for t in (7, 23, 40):
-        assert job.track_until(t) == 0
+        assert job.track_until(t).last_track_status == 0
     job.collect()
     return particles_dict(particles_in(pb))

@@ -56,17 +56,17 @@ def test_reached_turns_and_lost_sets_are_no_ops():
     eb = lattices.fodo_ring(n_cells=5)
     pb, _ = lattices.gaussian_beam(300, seed=2, sigma_x=3e-4, sigma_y=3e-4)
     job = _job(eb, pb)
-    assert job.track_until(4) == 0
+    assert job.track_until(4).last_track_status == 0
     job.collect()
     after4 = particles_dict(particles_in(pb))
     for t in (4, 3, 0):
-        assert job.track_until(t) == 0
+        assert job.track_until(t).last_track_status == 0
     job.collect()
     assert_particles_equal(particles_dict(particles_in(pb)), after4, what="repeated requests")
     lb, _ = lattices.gaussian_beam(70, seed=3, state=0)
     before = particles_dict(particles_in(lb))
     job = _job(eb, lb)
-    assert job.track_until(10) == 0
+    assert job.track_until(10).last_track_status == 0
     job.collect()
     assert_particles_equal(particles_dict(particles_in(lb)), before, what="all lost")
 
@@ -81,7 +81,7 @@ def test_at_element_offset_and_partial_waves(n):
     ob = _copy(pb)
     oracle_lib.track_until(ob, eb, 12)
     job = _job(eb, pb)
-    assert job.track_until(12) == 0
+    assert job.track_until(12).last_track_status == 0
     job.collect()
     got, want = particles_dict(particles_in(pb)), particles_dict(particles_in(ob))
     assert_particles_equal(got, want, what="offset start")
@@ -99,7 +99,7 @@ def test_non_finite_coordinates_are_lost_at_the_first_drift():
     ob = _copy(pb)
     oracle_lib.track_until(ob, eb, 2)
     job = _job(eb, pb)
-    assert job.track_until(2) == 0
+    assert job.track_until(2).last_track_status == 0
     job.collect()
     got = particles_dict(particles_in(pb))
     assert_particles_equal(got, particles_dict(particles_in(ob)), what="non-finite")
@@ -139,7 +139,7 @@ def test_selected_particle_sets_only():
     for k in (0, 2):
         oracle_lib.track_until(ob, eb, 5, index=k)
     job = _job(eb, cb, particle_set_indices=[0, 2])
-    assert job.track_until(5) == 0
+    assert job.track_until(5).last_track_status == 0
     job.collect()
     for k in (0, 2):
         assert_particles_equal(particles_dict(particles_in(cb, k)),

@@ -175,7 +175,7 @@ def test_fodo_ring_losses_in_steps_and_compaction():
                           ("pairs", pb4, "fma=0 ppl=2 turns_per_launch=3")):
         job = TrackJob(eb, buf, config_str=cfg)
         for t in (3, 7, 10):
-            assert job.track_until(t) == 0
+            assert job.track_until(t).last_track_status == 0
         job.collect()
         got[key] = _pdict(buf)
     want = _pdict(ob)
@@ -203,7 +203,7 @@ def test_two_particles_per_lane_match_one(fma):
     for key, buf, cfg in (("one", pa, "ppl=1"), ("two", pb, "ppl=2")):
         job = TrackJob(eb, buf, config_str="fma=%d %s" % (int(fma), cfg))
         for t in (4, 9, 20):
-            assert job.track_until(t) == 0
+            assert job.track_until(t).last_track_status == 0
         job.collect()
         res[key] = _pdict(buf)
     assert 50 < np.count_nonzero(res["one"]["state"] == 0) < 2800
@@ -221,7 +221,7 @@ def test_kitchen_sink_exact_bitwise(seed):
     ob = _copy(pb)
     st_o = oracle_lib.track_until(ob, eb, 5)
     job = _job(eb, pb)
-    st_g = job.track_until(5)
+    st_g = job.track_until(5).last_track_status
     job.collect()
     assert st_o == st_g == 0
     assert_particles_equal(_pdict(pb), _pdict(ob), what="kitchen sink")
@@ -414,13 +414,13 @@ def test_elem_by_elem_in_windows_matches_oracle():
     ob, cb = _copy(pb), _copy(pb)
     job = TrackJob(eb, pb, until_turn_elem_by_elem=T, config_str="fma=0")
     off = job.elem_by_elem_output_offset
-    assert job.track_elem_by_elem(T) == 0
+    assert job.track_elem_by_elem(T).last_track_status == 0
     job.collect()
     got = _pdict(job.output, off)
     # compact=0: no select pass, the live particles in rounds of one wave
     # per SIMD for the whole request
     job2 = TrackJob(eb, cb, until_turn_elem_by_elem=T, config_str="fma=0 compact=0")
-    assert job2.track_elem_by_elem(T) == 0
+    assert job2.track_elem_by_elem(T).last_track_status == 0
     job2.collect()
     got2 = _pdict(job2.output, job2.elem_by_elem_output_offset)
     ocb = CBuffer(max_slots=22 * n * E * T + 64)
