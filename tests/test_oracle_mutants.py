@@ -49,6 +49,20 @@ MUTANTS = {
                         "out = turn_off * np * ne + pid_off * ne + elem_off;"),
     "rf_multipole_sign": ("double const py_kick =  chi * dpy;",
                           "double const py_kick = -chi * dpy;"),
+    "rf_energy_cos_for_sin": ("dptr += sn * ( bal_n_kk * zre ) - ss * ( bal_s_kk * zim );",
+                              "dptr += cn * ( bal_n_kk * zre ) - ss * ( bal_s_kk * zim );"),
+    "rf_energy_sign": ("double const rfmultipole_energy = -q * ( ( k * p0c ) * dptr );",
+                       "double const rfmultipole_energy = q * ( ( k * p0c ) * dptr );"),
+    "rf_skew_phase_index": ("double const ps_kk = DEG2RAD * phase[ 2 * kk + 1 ] - ktau;",
+                            "double const ps_kk = DEG2RAD * phase[ 2 * kk ] - ktau;"),
+    "qgauss_exponent": ("e = OR_POW( u_plus, 1.0 / one_minus_q );",
+                        "e = OR_POW( u_plus, one_minus_q );"),
+    "qgauss_support": ("if( u_plus < 0.0 ) u_plus = 0.0;", ""),
+    "monitor_rolling_slot": ("store_idx = store_idx % m->num_stores;",
+                             "store_idx = ( store_idx + 1 ) % m->num_stores;"),
+    "monitor_particle_order": ("else                     out = pid_off * nstore + store_idx;",
+                               "else                     out = pid_off * m->num_stores + store_idx;"),
+    "ebe_rolling_wrap": ("out = out % capacity;", "out = ( out + np ) % capacity;"),
 }
 
 
