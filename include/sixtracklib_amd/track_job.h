@@ -113,6 +113,23 @@ ST_API NS(track_status_t) NS(TrackJob_track_elem_by_elem)( NS(TrackJobBase)* job
 ST_API NS(track_status_t) NS(TrackJob_track_line)( NS(TrackJobBase)* job,
     NS(buffer_size_t) begin_idx, NS(buffer_size_t) end_idx, bool finish_turn );
 
+/* No counterpart in the reference.  For callers that cannot allocate
+ * device memory while they track (tight serving loops, code that must not
+ * call hipMalloc): NS(TrackJob_track_until) with the same arguments,
+ * scheduling, kernels and results, but every scratch array of the call
+ * (status word, particle-set descriptors, counts, work queues,
+ * active-particle lists) lives in one static device array of the library
+ * (8.06 MiB on each GPU the library runs on, held until the process
+ * ends); the call allocates and frees nothing.  Limits: at most 64
+ * particle sets of together at most 2,097,152 particles (status -1 and a
+ * message in NS(Hip_last_error_str) beyond them).  The job itself (its
+ * particle, lattice and output copies) is made by the usual constructors.
+ * All calls on one GPU share that array, which keeps the last call's
+ * contents: calls that overlap in time on one GPU (from several host
+ * threads) race and must not be made. */
+ST_API NS(track_status_t) NS(TrackJob_track_until_noalloc)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) until_turn );
+
 /* track_job.h:188-244 */
 ST_API void NS(TrackJob_collect)( NS(TrackJobBase)* job );
 ST_API void NS(TrackJob_collect_detailed)( NS(TrackJobBase)* job, NS(track_job_collect_flag_t) flags );

@@ -126,10 +126,45 @@ struct Config
     }
 };
 
+// Seeds -> queues without allocating (S <= ST_QUEUE_PLAN_MAX_SEEDS):
+// queues = 8: greedy balance of particle counts over 8 queues (largest seed
+// first), one queue per XCD; blocks are handed to the XCDs round robin, so
+// block b runs on XCD b % 8 and starts on queue b % 8.  queues = 1: all
+// seeds in one queue, largest first.  qs receives S seed indices, qb the
+// nq + 1 queue starts.
+constexpr int64_t ST_QUEUE_PLAN_MAX_SEEDS = 4096;
+
+inline bool plan_queues( const int64_t* n_part, int64_t S, int nq, int32_t* qs, int32_t* qb )
+{
+    if( S > ST_QUEUE_PLAN_MAX_SEEDS || nq < 1 || nq > 8 ) return fail( "too many seeds for the queue plan" );
+    int32_t order[ ST_QUEUE_PLAN_MAX_SEEDS ];
+    for( int64_t s = 0; s < S; ++s ) order[ s ] = ( int32_t )s;
+    std::stable_sort( order, order + S,
+                      [ & ]( int32_t a, int32_t b ) { return n_part[ a ] > n_part[ b ]; } );
+    int32_t queue_of[ ST_QUEUE_PLAN_MAX_SEEDS ];
+    int64_t load[ 8 ] = { 0, 0, 0, 0, 0, 0, 0, 0 };
+    for( int64_t r = 0; r < S; ++r )
+    {
+        int k = 0;
+        for( int j = 1; j < nq; ++j ) if( load[ j ] < load[ k ] ) k = j;
+        queue_of[ r ] = k;
+        load[ k ] += n_part[ order[ r ] ];
+    }
+    int32_t n = 0;
+    for( int k = 0; k < nq; ++k )
+    {
+        qb[ k ] = n;
+        for( int64_t r = 0; r < S; ++r ) if( queue_of[ r ] == k ) qs[ n++ ] = order[ r ];
+    }
+    qb[ nq ] = n;
+    return true;
+}
+
 struct BatchRunner
 {
     Config cfg;
     int n_cu = 0;
+    bool borrowed = false;             // arrays carved from a caller's region (init_borrowed)
     std::vector< int64_t > n_part, n_elem;
     int64_t total_particles = 0, max_particles = 0;
 
@@ -155,11 +190,15 @@ struct BatchRunner
     {
         if( gexec ) ( void )hipGraphExecDestroy( gexec );
         gexec = nullptr; gkey.clear();
-        for( void* p : { ( void* )d_seeds, ( void* )d_lists, ( void* )d_counts, ( void* )d_qseeds,
-                         ( void* )d_qbegin, ( void* )d_qnext, ( void* )d_split, ( void* )d_progress } )
-            if( p ) ( void )hipFree( p );
-        if( h_counts ) ( void )hipHostFree( h_counts );
-        if( h_split ) ( void )hipHostFree( h_split );
+        if( !borrowed )
+        {
+            for( void* p : { ( void* )d_seeds, ( void* )d_lists, ( void* )d_counts, ( void* )d_qseeds,
+                             ( void* )d_qbegin, ( void* )d_qnext, ( void* )d_split, ( void* )d_progress } )
+                if( p ) ( void )hipFree( p );
+            if( h_counts ) ( void )hipHostFree( h_counts );
+            if( h_split ) ( void )hipHostFree( h_split );
+        }
+        borrowed = false;
         d_seeds = nullptr; d_lists = d_counts = d_qseeds = d_qbegin = d_qnext = h_counts = nullptr;
         d_split = h_split = nullptr;
         d_progress = nullptr;
@@ -214,33 +253,92 @@ struct BatchRunner
         return build_queues( stream );
     }
 
-    // Seeds -> queues.  queues=8: greedy balance of particle counts over 8
-    // queues (largest seed first), one queue per XCD: blocks are handed to
-    // the XCDs round robin, so block b runs on XCD b % 8 and starts on queue
-    // b % 8.  queues=1: all seeds in one queue, largest first.
+    // No-allocation set-up (the job's NS(TrackJob_track_until_noalloc)):
+    // every device array of the runner is carved out of ``d_region``
+    // (region_bytes, 256-byte aligned, contents arbitrary on entry: what
+    // must start at zero is cleared by the reset kernel before use) and the
+    // host read-back arrays out of ``h_region`` (pinned, host_ints(S)
+    // int32).  ``h_stage`` (pinned, stage_bytes(S)) stages the seed
+    // descriptors and queues for their upload; it must stay untouched until
+    // the stream has run the copies.  n_part / n_elem must already have
+    // capacity S.  Nothing is allocated or freed.
+    static constexpr size_t a256( size_t n ) { return ( n + 255u ) & ~( size_t )255u; }
+    static constexpr size_t region_bytes_for( int64_t S, int64_t total )
+    {
+        size_t const s = ( size_t )std::max< int64_t >( S, 1 );
+        return a256( s * sizeof( st_batch_seed_t ) ) + a256( 2 * s * 4 ) + a256( s * 4 ) +
+               a256( 9 * 4 ) + a256( 8 * 4 ) + a256( ( 2 * s + 1 ) * 4 ) + a256( 8 ) +
+               a256( ( size_t )std::max< int64_t >( total, 1 ) * 4 );
+    }
+    static size_t host_ints( int64_t S ) { return ( size_t )( 4 * std::max< int64_t >( S, 1 ) + 1 ); }
+    static size_t stage_bytes( int64_t S )
+    {
+        size_t const s = ( size_t )std::max< int64_t >( S, 1 );
+        return a256( s * sizeof( st_batch_seed_t ) ) + a256( ( s + 9 ) * 4 );
+    }
+
+    bool init_borrowed( const st_dev_particles_t* views, const uint64_t* lat, int64_t n_elem_per_set,
+                        int64_t S, unsigned char* d_region, size_t region_bytes, int32_t* h_region,
+                        unsigned char* h_stage, int device_cu, hipStream_t stream )
+    {
+        release();
+        borrowed = true;
+        n_cu = device_cu;
+        if( S < 1 || S > ST_QUEUE_PLAN_MAX_SEEDS ) return fail( "no-allocation tracking: bad set count" );
+        if( ( int64_t )n_part.capacity() < S || ( int64_t )n_elem.capacity() < S )
+            return fail( "no-allocation tracking: runner not prepared" );
+        for( int64_t s = 0; s < S; ++s )
+        {
+            if( views[ s ].n < 0 || views[ s ].n >= ( int64_t )INT32_MAX ) return fail( "particle set too large" );
+            n_part.push_back( views[ s ].n );
+            n_elem.push_back( n_elem_per_set );
+            total_particles += views[ s ].n;
+            max_particles = std::max( max_particles, views[ s ].n );
+        }
+        if( region_bytes_for( S, total_particles ) > region_bytes )
+            return fail( "no-allocation tracking: particle sets exceed the scratch region" );
+        unsigned char* c = d_region;
+        auto carve = [ & ]( size_t n ) { unsigned char* r = c; c += a256( n ); return r; };
+        d_seeds = ( st_batch_seed_t* )carve( S * sizeof( st_batch_seed_t ) );
+        d_counts = ( int32_t* )carve( 2 * S * 4 );
+        d_qseeds = ( int32_t* )carve( S * 4 );
+        d_qbegin = ( int32_t* )carve( 9 * 4 );
+        d_qnext = ( int32_t* )carve( 8 * 4 );
+        d_split = ( int32_t* )carve( ( 2 * S + 1 ) * 4 );
+        d_progress = ( unsigned long long* )carve( 8 );
+        d_lists = ( int32_t* )carve( ( size_t )std::max< int64_t >( total_particles, 1 ) * 4 );
+        h_counts = h_region;
+        h_split = h_region + 2 * S;
+        st_batch_seed_t* hs = ( st_batch_seed_t* )h_stage;
+        int32_t* hq = ( int32_t* )( h_stage + a256( S * sizeof( st_batch_seed_t ) ) );
+        int64_t list_off = 0;
+        for( int64_t s = 0; s < S; ++s )
+        {
+            hs[ s ].P = views[ s ];
+            hs[ s ].lat = lat;
+            hs[ s ].n_elem = n_elem_per_set;
+            hs[ s ].list = d_lists + list_off;
+            list_off += n_part[ s ];
+        }
+        if( !plan_queues( n_part.data(), S, cfg.queues, hq, hq + S ) ) return false;
+        return hip_ok( hipMemcpyAsync( d_seeds, hs, S * sizeof( st_batch_seed_t ),
+                                       hipMemcpyHostToDevice, stream ), "upload seeds" ) &&
+               hip_ok( hipMemcpyAsync( d_qseeds, hq, S * 4, hipMemcpyHostToDevice, stream ),
+                       "upload queues" ) &&
+               hip_ok( hipMemcpyAsync( d_qbegin, hq + S, ( cfg.queues + 1 ) * 4, hipMemcpyHostToDevice,
+                                       stream ), "upload queues" );
+    }
+
+    // Seeds -> queues (plan_queues), uploaded to fresh queue arrays.
     bool build_queues( hipStream_t stream )
     {
+        if( borrowed ) return fail( "queues of a no-allocation runner are set at its set-up" );
         if( gexec ) { ( void )hipGraphExecDestroy( gexec ); gexec = nullptr; gkey.clear(); }
         int64_t const S = n_seeds();
         int const nq = cfg.queues;
-        std::vector< int64_t > order( S );
-        std::iota( order.begin(), order.end(), 0 );
-        std::stable_sort( order.begin(), order.end(),
-                          [ & ]( int64_t a, int64_t b ) { return n_part[ a ] > n_part[ b ]; } );
-        std::vector< std::vector< int32_t > > q( nq );
-        std::vector< int64_t > load( nq, 0 );
-        for( int64_t s : order )
-        {
-            int const k = ( int )( std::min_element( load.begin(), load.end() ) - load.begin() );
-            q[ k ].push_back( ( int32_t )s );
-            load[ k ] += n_part[ s ];
-        }
-        std::vector< int32_t > qs, qb( 1, 0 );
-        for( int k = 0; k < nq; ++k )
-        {
-            qs.insert( qs.end(), q[ k ].begin(), q[ k ].end() );
-            qb.push_back( ( int32_t )qs.size() );
-        }
+        std::vector< int32_t > qs( std::max< int64_t >( S, 1 ) ), qb( nq + 1 );
+        if( !plan_queues( n_part.data(), S, nq, qs.data(), qb.data() ) ) return false;
+        qs.resize( S );
         if( d_qseeds ) { ( void )hipFree( d_qseeds ); d_qseeds = nullptr; }
         if( d_qbegin ) { ( void )hipFree( d_qbegin ); d_qbegin = nullptr; }
         if( !hip_ok( hipMalloc( &d_qseeds, std::max< size_t >( qs.size(), 1 ) * sizeof( int32_t ) ),

@@ -20,6 +20,7 @@
 #include <climits>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -31,9 +32,54 @@
 #include "batch.h"
 #include "batch_runner.h"
 
+// Scratch memory of NS(TrackJob_track_until_noalloc): the scheduler's
+// per-call device arrays (status word, seed descriptors, counts, queues,
+// active-particle lists) for at most ST_NOALLOC_MAX_SETS particle sets of
+// together at most ST_NOALLOC_MAX_PARTICLES particles.  Every GPU the
+// library runs on holds one copy until the process ends.  In host code the
+// name ``scratch`` is only a stand-in for the symbol: the device address
+// of each GPU's copy comes from hipGetSymbolAddress (scratch_address).
+constexpr int64_t ST_NOALLOC_MAX_SETS = 64;
+constexpr int64_t ST_NOALLOC_MAX_PARTICLES = int64_t( 1 ) << 21;
+constexpr size_t ST_NOALLOC_STATUS_BYTES = 256;
+constexpr size_t ST_NOALLOC_SCRATCH_BYTES = ( size_t( 8 ) << 20 ) + ( size_t( 64 ) << 10 );
+
+static_assert( ST_NOALLOC_STATUS_BYTES + st_batch::BatchRunner::region_bytes_for(
+                   ST_NOALLOC_MAX_SETS, ST_NOALLOC_MAX_PARTICLES ) <= ST_NOALLOC_SCRATCH_BYTES,
+               "scratch memory holds the largest no-allocation call" );
+
+static __device__ __align__( 256 ) unsigned char scratch[ ST_NOALLOC_SCRATCH_BYTES ];
+
 namespace
 {
 thread_local std::string g_err;
+
+constexpr int ST_NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_address[ ST_NOALLOC_MAX_DEVICES ] = {};
+
+// Device address of ``scratch`` on ``device``: looked up once per device,
+// with that device current, at the set-up of the first job on it.  nullptr
+// (and the error message set) when the lookup fails.
+void* scratch_address( int device, bool lookup )
+{
+    if( device < 0 || device >= ST_NOALLOC_MAX_DEVICES ) return nullptr;
+    std::lock_guard< std::mutex > lock( g_scratch_mutex );
+    if( g_scratch_address[ device ] != nullptr || !lookup ) return g_scratch_address[ device ];
+    int current = -1;
+    if( hipGetDevice( &current ) != hipSuccess ) return nullptr;
+    if( current != device && hipSetDevice( device ) != hipSuccess ) return nullptr;
+    void* ptr = nullptr;
+    hipError_t const e = hipGetSymbolAddress( ( void** )&ptr, HIP_SYMBOL( scratch ) );
+    if( current != device ) ( void )hipSetDevice( current );
+    if( e != hipSuccess || ptr == nullptr )
+    {
+        g_err = std::string( "hipGetSymbolAddress(scratch): " ) + hipGetErrorString( e );
+        return nullptr;
+    }
+    g_scratch_address[ device ] = ptr;
+    return ptr;
+}
 
 bool hip_ok( hipError_t e, const char* what )
 {
@@ -183,6 +229,12 @@ struct NS(TrackJobBase)
     int waves_per_cu = 0;                // resident waves per CU (0: the kernel's occupancy)
     st_batch::BatchRunner runner;        // scheduling of the lean until path
     bool runner_ready = false;
+    // NS(TrackJob_track_until_noalloc): a runner on the device's scratch
+    // memory, with its pinned host arrays made at set-up
+    st_batch::BatchRunner na_runner;
+    int32_t* na_host = nullptr;
+    unsigned char* na_stage = nullptr;
+    int n_cu = 0;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     float last_kernel_ms = 0.0f;
     std::vector< st_dev_particles_t > views;
@@ -253,6 +305,10 @@ struct NS(TrackJobBase)
         if( d_status ) ( void )hipFree( d_status );
         runner.release();
         runner_ready = false;
+        na_runner.release();
+        if( na_host ) ( void )hipHostFree( na_host );
+        if( na_stage ) ( void )hipHostFree( na_stage );
+        na_host = nullptr; na_stage = nullptr;
         if( h_status ) ( void )hipHostFree( h_status );
         if( ev_start ) ( void )hipEventDestroy( ev_start );
         if( ev_stop ) ( void )hipEventDestroy( ev_stop );
@@ -433,6 +489,21 @@ struct NS(TrackJobBase)
             !hip_ok( hipEventCreate( &ev_stop ), "hipEventCreate" ) ) return false;
         if( !hip_ok( hipMalloc( &d_status, sizeof( int32_t ) ), "hipMalloc status" ) ) return false;
         if( !hip_ok( hipHostMalloc( &h_status, sizeof( int32_t ) ), "hipHostMalloc" ) ) return false;
+        if( !hip_ok( hipDeviceGetAttribute( &n_cu, hipDeviceAttributeMultiprocessorCount, device ),
+                     "hipDeviceGetAttribute" ) ) return false;
+        // what the no-allocation entry point needs besides the device's
+        // scratch memory, made here so that it allocates nothing itself
+        if( ( int64_t )sets.size() <= ST_NOALLOC_MAX_SETS )
+        {
+            int64_t const S = ( int64_t )sets.size();
+            na_runner.n_part.reserve( S );
+            na_runner.n_elem.reserve( S );
+            if( !hip_ok( hipHostMalloc( &na_host, st_batch::BatchRunner::host_ints( S ) * sizeof( int32_t ) ),
+                         "hipHostMalloc" ) ||
+                !hip_ok( hipHostMalloc( &na_stage, st_batch::BatchRunner::stage_bytes( S ) ),
+                         "hipHostMalloc" ) ) return false;
+            if( scratch_address( device, true ) == nullptr ) return false;
+        }
         pbuf_bytes = NS(Buffer_get_size)( pbuf );
         if( !hip_ok( hipMalloc( &d_pbuf, pbuf_bytes ), "hipMalloc particles" ) ) return false;
         if( !upload_image( pbuf, d_pbuf, pbuf_bytes ) ) return false;
@@ -484,23 +555,29 @@ struct NS(TrackJobBase)
     // reads the counts back between launches; balance=0 hands every wave
     // tiles until none are left in each chunk.  compact=0 runs one launch
     // with one lane per particle.
+    st_batch::Config runner_config() const
+    {
+        st_batch::Config c;
+        c.fma = fma;
+        c.block = ( block == 64 || block == 128 ) ? block : 256;
+        c.turns_per_launch = turns_per_launch;
+        c.queues = queues;
+        c.heavy = lat_full;
+        c.ppl = lat_full ? 1 : ppl;
+        c.balance = balance;
+        c.wave_prio = wave_prio;
+        // resident waves per CU: 8 per SIMD with one particle per lane, 5
+        // with two (the kernels' register budgets)
+        c.waves_per_cu = c.ppl == 2 ? 20 : 32;
+        if( waves_per_cu > 0 ) c.waves_per_cu = std::min( waves_per_cu, c.waves_per_cu );
+        c.graph = false;
+        return c;
+    }
+
     bool ensure_runner()
     {
         if( runner_ready ) return true;
-        runner.cfg = st_batch::Config();
-        runner.cfg.fma = fma;
-        runner.cfg.block = ( block == 64 || block == 128 ) ? block : 256;
-        runner.cfg.turns_per_launch = turns_per_launch;
-        runner.cfg.queues = queues;
-        runner.cfg.heavy = lat_full;
-        runner.cfg.ppl = lat_full ? 1 : ppl;
-        runner.cfg.balance = balance;
-        runner.cfg.wave_prio = wave_prio;
-        // resident waves per CU: 8 per SIMD with one particle per lane, 5
-        // with two (the kernels' register budgets)
-        runner.cfg.waves_per_cu = runner.cfg.ppl == 2 ? 20 : 32;
-        if( waves_per_cu > 0 ) runner.cfg.waves_per_cu = std::min( waves_per_cu, runner.cfg.waves_per_cu );
-        runner.cfg.graph = false;
+        runner.cfg = runner_config();
         std::vector< st_batch_seed_t > seeds;
         for( const st_dev_particles_t& v : views )
         {
@@ -538,6 +615,77 @@ struct NS(TrackJobBase)
             ok = ok && hip_ok( err, "launch track_until" );
         }
         return end_status( ok );
+    }
+
+    // track_until with every scratch array of the call in the device's
+    // ``scratch`` memory: the same scheduling and kernels, nothing
+    // allocated or freed.  Calls from several jobs on one device share that
+    // memory; calls that overlap in time on one device race.
+    int32_t track_until_noalloc( int64_t until_turn )
+    {
+        int64_t const S = ( int64_t )views.size();
+        int64_t total = 0;
+        for( const st_dev_particles_t& v : views ) total += v.n;
+        unsigned char* const base = ( unsigned char* )scratch_address( device, false );
+        if( base == nullptr || na_host == nullptr || na_stage == nullptr )
+        { g_err = "no-allocation tracking: the device's scratch memory is not set up"; last_status = -1; return -1; }
+        if( S > ST_NOALLOC_MAX_SETS || total > ST_NOALLOC_MAX_PARTICLES )
+        {
+            char m[ 160 ];
+            std::snprintf( m, sizeof( m ), "no-allocation tracking takes at most %lld particle sets "
+                           "of together %lld particles", ( long long )ST_NOALLOC_MAX_SETS,
+                           ( long long )ST_NOALLOC_MAX_PARTICLES );
+            g_err = m;
+            last_status = -1;
+            return -1;
+        }
+        int32_t* const status = ( int32_t* )base;
+        if( !hip_ok( hipMemsetAsync( status, 0, sizeof( int32_t ), stream ), "hipMemsetAsync" ) ||
+            !hip_ok( hipEventRecord( ev_start, stream ), "hipEventRecord" ) )
+        { last_status = -1; return -1; }
+        bool ok = true;
+        if( compact && n_elem > 0 )
+        {
+            na_runner.cfg = runner_config();
+            ok = na_runner.init_borrowed( views.data(), d_lat, n_elem, S,
+                                          base + ST_NOALLOC_STATUS_BYTES,
+                                          ST_NOALLOC_SCRATCH_BYTES - ST_NOALLOC_STATUS_BYTES,
+                                          na_host, na_stage, n_cu, stream );
+            if( ok && balance ) ok = na_runner.run_balanced( until_turn, status, stream );
+            else if( ok )
+            {
+                int64_t n_active = 0, min_turn = 0;
+                ok = na_runner.select_sync( until_turn, stream, n_active, min_turn );
+                if( ok && n_active > 0 )
+                {
+                    int64_t const k = na_runner.chunk_turns( n_active );
+                    ok = na_runner.run( until_turn, ( until_turn - min_turn + k - 1 ) / k, k, status,
+                                        stream );
+                }
+            }
+        }
+        else
+            for( const st_dev_particles_t& v : views )
+            {
+                hipError_t const err = fma
+                    ? st_launch_track_until_fma( &v, d_lat, n_elem, until_turn, lat_full, status, block, stream )
+                    : st_launch_track_until_exact( &v, d_lat, n_elem, until_turn, lat_full, status, block, stream );
+                ok = ok && hip_ok( err, "launch track_until" );
+            }
+        if( !ok ||
+            !hip_ok( hipEventRecord( ev_stop, stream ), "hipEventRecord" ) ||
+            !hip_ok( hipMemcpyAsync( h_status, status, sizeof( int32_t ), hipMemcpyDeviceToHost, stream ),
+                     "status D2H" ) ||
+            !hip_ok( hipStreamSynchronize( stream ), "hipStreamSynchronize" ) )
+        {
+            ( void )hipStreamSynchronize( stream );
+            last_status = -1;
+            return -1;
+        }
+        float ms = 0.0f;
+        last_kernel_ms = ( hipEventElapsedTime( &ms, ev_start, ev_stop ) == hipSuccess ) ? ms : -1.0f;
+        last_status = *h_status;
+        return last_status;
     }
 
     int32_t track_ebe( int64_t until_turn )
@@ -689,6 +837,11 @@ void NS(TrackJob_delete)( NS(TrackJobBase)* job ) { delete job; }
 NS(track_status_t) NS(TrackJob_track_until)( NS(TrackJobBase)* job, NS(buffer_size_t) until_turn )
 {
     return ( job && job->d_pbuf ) ? job->track_until( ( int64_t )until_turn ) : -1;
+}
+
+NS(track_status_t) NS(TrackJob_track_until_noalloc)( NS(TrackJobBase)* job, NS(buffer_size_t) until_turn )
+{
+    return ( job && job->d_pbuf ) ? job->track_until_noalloc( ( int64_t )until_turn ) : -1;
 }
 
 NS(track_status_t) NS(TrackJob_track_elem_by_elem)( NS(TrackJobBase)* job, NS(buffer_size_t) until_turn )
@@ -990,5 +1143,6 @@ NS(arch_status_t) NS(TrackJob_assign_addresses)( NS(TrackJobBase)* j,
     return ( j && j->assign( false, dest_buffer_id, src_buffer_id ) )
         ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
 }
+
 
 }  // extern "C"

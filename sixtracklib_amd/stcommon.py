@@ -113,6 +113,7 @@ _fn("st_TrackJob_new_detailed", st_TrackJob_p,
      st_Buffer_p, st_buffer_size_t, ct.c_char_p])
 _fn("st_TrackJob_delete", None, [st_TrackJob_p])
 _fn("st_TrackJob_track_until", st_track_status_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_track_until_noalloc", st_track_status_t, [st_TrackJob_p, st_buffer_size_t])
 _fn("st_TrackJob_track_elem_by_elem", st_track_status_t, [st_TrackJob_p, st_buffer_size_t])
 _fn("st_TrackJob_track_line", st_track_status_t,
     [st_TrackJob_p, st_buffer_size_t, st_buffer_size_t, ct.c_bool])
