@@ -26,6 +26,19 @@ NS(Particles);
 #define ST_PARTICLES_NUM_REAL_FIELDS  17
 #define ST_PARTICLES_NUM_INDEX_FIELDS 4
 
+/* particles/particles_addr.h:27-55: the addresses of the 21 field arrays
+ * of one particle set (object type ST_OBJECT_TYPE_PARTICLES_ADDR).  A track
+ * job fills them with the addresses of its device copy. */
+typedef struct NS(ParticlesAddr)
+{
+    NS(particle_num_elements_t) num_particles;
+    NS(buffer_addr_t) q0_addr, mass0_addr, beta0_addr, gamma0_addr, p0c_addr;
+    NS(buffer_addr_t) s_addr, x_addr, y_addr, px_addr, py_addr, zeta_addr;
+    NS(buffer_addr_t) psigma_addr, delta_addr, rpp_addr, rvv_addr, chi_addr, charge_ratio_addr;
+    NS(buffer_addr_t) particle_id_addr, at_element_id_addr, at_turn_addr, state_addr;
+}
+NS(ParticlesAddr);
+
 /* particles.h: NS(Particles_new) -- zero-initialised (state = 0). */
 ST_API NS(Particles)* NS(Particles_new)( NS(Buffer)* buffer,
     NS(particle_num_elements_t) num_particles );

@@ -27,15 +27,19 @@
  *                         (128-particle tiles; results identical)
  *   wave_prio=0|1         waves with more turns left in their tile get
  *                         issue priority (1; results identical)
+ *   staging=scalar|lds    lattice read through scalar loads by every wave
+ *                         (scalar) or copied piece by piece into LDS by
+ *                         256-lane workgroups that walk it together (lds;
+ *                         one particle per lane; results identical)
  *   balance=0|1           0: chunks of turns, every wave taking tiles until
  *                         none are left (graph-captured); 1: launches of one
  *                         tile per resident wave, the particles furthest
  *                         behind first, the host reading the counts back
  *                         between launch groups (no graph; results
  *                         identical)
- * NS(TrackBatch_autotune) picks block, waves_per_cu and queues for one
- * particle per lane by timing short trials on the batch itself, the three
- * fastest twice (state restored afterwards).
+ * NS(TrackBatch_autotune) picks block, waves_per_cu, queues, particles per
+ * lane and lattice staging by timing short trials on the batch itself, the
+ * three fastest twice (state restored afterwards).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_BATCH_H__
 #define SIXTRACKLIB_AMD_TRACK_BATCH_H__
@@ -67,8 +71,14 @@ ST_API int NS(TrackBatch_set_config)( NS(TrackBatch)* batch, char const* config_
 /* writes the effective configuration (NUL-terminated); returns its length */
 ST_API int64_t NS(TrackBatch_get_config)( const NS(TrackBatch)* batch, char* out, int64_t out_len );
 /* times each candidate configuration for trial_turns turns on the batch,
- * keeps the fastest; 0 on success */
+ * after 10 x trial_turns turns tracked with the current configuration (the
+ * trials see the batch past its early losses), keeps the fastest and
+ * restores the batch state; 0 on success */
 ST_API int NS(TrackBatch_autotune)( NS(TrackBatch)* batch, int64_t trial_turns );
+/* as NS(TrackBatch_autotune) with the trials starting after skip_turns
+ * turns (0: at the current state) */
+ST_API int NS(TrackBatch_autotune_window)( NS(TrackBatch)* batch, int64_t skip_turns,
+    int64_t trial_turns );
 
 ST_API int64_t NS(TrackBatch_num_seeds)( const NS(TrackBatch)* batch );
 ST_API int64_t NS(TrackBatch_total_particles)( const NS(TrackBatch)* batch );

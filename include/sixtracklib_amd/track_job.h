@@ -85,6 +85,40 @@ typedef struct NS(AssignAddressItem)
 }
 NS(AssignAddressItem);
 
+/* buffer/assign_address_item.h:27-32 */
+typedef struct NS(AssignAddressItemDestSrcPair)
+{
+    NS(buffer_size_t) dest_buffer_id;
+    NS(buffer_size_t) src_buffer_id;
+}
+NS(AssignAddressItemDestSrcPair);
+typedef NS(AssignAddressItemDestSrcPair) NS(TrackJobDestSrcBufferIds);
+
+/* output/elem_by_elem_config.h:33-53: the element-by-element store
+ * (order: only turn-element-particle, value 0, exists) */
+typedef int64_t NS(elem_by_elem_order_t);
+#define ST_ELEM_BY_ELEM_ORDER_TURN_ELEM_PARTICLES ( ( NS(elem_by_elem_order_t) )0 )
+#define ST_ELEM_BY_ELEM_ORDER_DEFAULT ST_ELEM_BY_ELEM_ORDER_TURN_ELEM_PARTICLES
+typedef struct NS(ElemByElemConfig)
+{
+    NS(elem_by_elem_order_t) order;
+    int64_t num_particles_to_store, num_elements_to_store, num_turns_to_store;
+    int64_t min_particle_id, min_element_id, min_turn;
+    int64_t max_particle_id, max_element_id, max_turn;
+    int64_t is_rolling;
+    NS(buffer_addr_t) out_store_addr;
+}
+NS(ElemByElemConfig);
+
+/* track_job.h:123-135: the device id in a configuration string
+ * ("device=N", or a leading "P.D" platform.device pair as the reference's
+ * OpenCL ids); the lower-case arch string.  0 on success. */
+ST_API int NS(TrackJob_extract_device_id_str)( char const* config_str, char* device_id_str,
+    NS(buffer_size_t) max_device_id_str_len );
+ST_API int NS(TrackJob_sanitize_arch_str_inplace)( char* arch_str, NS(buffer_size_t) max_arch_str_len );
+ST_API int NS(TrackJob_sanitize_arch_str)( char const* arch_str, char* sanitized_arch_str,
+    NS(buffer_size_t) max_arch_str_len );
+
 /* track_job.h:139 */
 ST_API NS(TrackJobBase)* NS(TrackJob_create)( char const* arch, char const* config_str );
 /* track_job.h:143 */
@@ -155,6 +189,31 @@ ST_API void NS(TrackJob_push_particles)( NS(TrackJobBase)* job );
 ST_API void NS(TrackJob_push_beam_elements)( NS(TrackJobBase)* job );
 ST_API void NS(TrackJob_push_output)( NS(TrackJobBase)* job );
 
+/* track_job.h:206-290: particle addresses.  A job keeps one
+ * NS(ParticlesAddr) per Particles object of its particle buffer; fetch
+ * fills them with the device addresses of the job's copy (valid until the
+ * job is reset or deleted), clear zeroes them.  They are made on the host:
+ * collect and push of the addresses have nothing to move. */
+ST_API void NS(TrackJob_collect_particles_addresses)( NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_push_particles_addresses)( NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_can_fetch_particles_addr)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_has_particles_addr)( const NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_fetch_particle_addresses)( NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_fetch_particles_addr)( NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_clear_particle_addresses)( NS(TrackJobBase)* job,
+    NS(arch_size_t) index );
+ST_API NS(arch_status_t) NS(TrackJob_clear_particles_addr)( NS(TrackJobBase)* job,
+    NS(buffer_size_t) particle_set_index );
+ST_API NS(arch_status_t) NS(TrackJob_clear_all_particle_addresses)( NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_clear_all_particles_addr)( NS(TrackJobBase)* job );
+ST_API NS(ParticlesAddr) const* NS(TrackJob_particle_addresses)( const NS(TrackJobBase)* job,
+    NS(arch_size_t) index );
+ST_API NS(ParticlesAddr) const* NS(TrackJob_particles_addr)( const NS(TrackJobBase)* job,
+    NS(buffer_size_t) particle_set_index );
+ST_API NS(Buffer)* NS(TrackJob_get_particles_addr_buffer)( NS(TrackJobBase)* job );
+ST_API NS(Buffer) const* NS(TrackJob_get_const_particles_addr_buffer)( const NS(TrackJobBase)* job );
+ST_API NS(Buffer) const* NS(TrackJob_particles_addr_buffer)( const NS(TrackJobBase)* job );
+
 /* track_job.h:300-336 */
 ST_API void NS(TrackJob_clear)( NS(TrackJobBase)* job );
 ST_API bool NS(TrackJob_reset)( NS(TrackJobBase)* job, NS(Buffer)* particles_buffer,
@@ -167,6 +226,9 @@ ST_API bool NS(TrackJob_reset_detailed)( NS(TrackJobBase)* job,
     NS(buffer_size_t) const* particle_set_indices_begin,
     NS(Buffer)* beam_elements_buffer, NS(Buffer)* output_buffer,
     NS(buffer_size_t) until_turn_elem_by_elem );
+ST_API bool NS(TrackJob_reset_particle_set)( NS(TrackJobBase)* job,
+    NS(Buffer)* particles_buffer, NS(buffer_size_t) particle_set_index,
+    NS(Buffer)* beam_elements_buffer, NS(Buffer)* output_buffer );
 ST_API bool NS(TrackJob_select_particle_set)( NS(TrackJobBase)* job,
     NS(buffer_size_t) particle_set_index );
 ST_API bool NS(TrackJob_assign_output_buffer)( NS(TrackJobBase)* job,
@@ -184,6 +246,10 @@ ST_API NS(buffer_size_t) const* NS(TrackJob_get_particle_set_indices_begin)( con
 ST_API NS(buffer_size_t) const* NS(TrackJob_get_particle_set_indices_end)( const NS(TrackJobBase)* job );
 ST_API NS(buffer_size_t) NS(TrackJob_get_particle_set_index)( const NS(TrackJobBase)* job, NS(buffer_size_t) n );
 ST_API NS(buffer_size_t) NS(TrackJob_get_total_num_particles_in_sets)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) const* NS(TrackJob_get_num_particles_in_sets_begin)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) const* NS(TrackJob_get_num_particles_in_sets_end)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_num_particles_in_set)( const NS(TrackJobBase)* job,
+    NS(buffer_size_t) n );
 ST_API NS(particle_index_t) NS(TrackJob_get_min_particle_id)( const NS(TrackJobBase)* job );
 ST_API NS(particle_index_t) NS(TrackJob_get_max_particle_id)( const NS(TrackJobBase)* job );
 ST_API NS(particle_index_t) NS(TrackJob_get_min_element_id)( const NS(TrackJobBase)* job );
@@ -191,7 +257,9 @@ ST_API NS(particle_index_t) NS(TrackJob_get_max_element_id)( const NS(TrackJobBa
 ST_API NS(particle_index_t) NS(TrackJob_get_min_initial_turn_id)( const NS(TrackJobBase)* job );
 ST_API NS(particle_index_t) NS(TrackJob_get_max_initial_turn_id)( const NS(TrackJobBase)* job );
 ST_API NS(Buffer)* NS(TrackJob_get_particles_buffer)( NS(TrackJobBase)* job );
+ST_API NS(Buffer) const* NS(TrackJob_get_const_particles_buffer)( const NS(TrackJobBase)* job );
 ST_API NS(Buffer)* NS(TrackJob_get_beam_elements_buffer)( NS(TrackJobBase)* job );
+ST_API NS(Buffer) const* NS(TrackJob_get_const_beam_elements_buffer)( const NS(TrackJobBase)* job );
 ST_API bool NS(TrackJob_has_output_buffer)( const NS(TrackJobBase)* job );
 ST_API bool NS(TrackJob_owns_output_buffer)( const NS(TrackJobBase)* job );
 ST_API bool NS(TrackJob_has_elem_by_elem_output)( const NS(TrackJobBase)* job );
@@ -201,6 +269,27 @@ ST_API NS(buffer_size_t) NS(TrackJob_get_elem_by_elem_output_buffer_offset)( con
 ST_API NS(buffer_size_t) NS(TrackJob_get_num_elem_by_elem_turns)( const NS(TrackJobBase)* job );
 ST_API NS(buffer_size_t) NS(TrackJob_get_num_beam_monitors)( const NS(TrackJobBase)* job );
 ST_API NS(Buffer)* NS(TrackJob_get_output_buffer)( NS(TrackJobBase)* job );
+ST_API NS(Buffer) const* NS(TrackJob_get_const_output_buffer)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_has_beam_monitors)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) const* NS(TrackJob_get_beam_monitor_indices_begin)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) const* NS(TrackJob_get_beam_monitor_indices_end)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_get_beam_monitor_index)( const NS(TrackJobBase)* job,
+    NS(buffer_size_t) n );
+/* the element-by-element configuration of the job's output (NULL when
+ * the job stores no element-by-element output); out_store_addr is the
+ * host address of the store in the output buffer */
+ST_API bool NS(TrackJob_has_elem_by_elem_config)( const NS(TrackJobBase)* job );
+ST_API NS(ElemByElemConfig) const* NS(TrackJob_get_elem_by_elem_config)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_is_elem_by_elem_config_rolling)( const NS(TrackJobBase)* job );
+ST_API bool NS(TrackJob_get_default_elem_by_elem_config_rolling_flag)( const NS(TrackJobBase)* job );
+/* takes effect at the next reset, as the reference's */
+ST_API void NS(TrackJob_set_default_elem_by_elem_config_rolling_flag)( NS(TrackJobBase)* job,
+    bool is_rolling_flag );
+ST_API NS(elem_by_elem_order_t) NS(TrackJob_get_elem_by_elem_config_order)( const NS(TrackJobBase)* job );
+ST_API NS(elem_by_elem_order_t) NS(TrackJob_get_default_elem_by_elem_config_order)(
+    const NS(TrackJobBase)* job );
+ST_API void NS(TrackJob_set_default_elem_by_elem_config_order)( NS(TrackJobBase)* job,
+    NS(elem_by_elem_order_t) order );
 ST_API NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* job );
 
 /* Timing and build-variant introspection (no reference counterpart):
@@ -210,6 +299,8 @@ ST_API NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* job 
 ST_API double NS(TrackJob_last_kernel_time_ms)( const NS(TrackJobBase)* job );
 ST_API int NS(TrackJob_uses_fma)( const NS(TrackJobBase)* job );
 ST_API int NS(TrackJob_uses_full_kernel)( const NS(TrackJobBase)* job );
+/* threads per block of the job's launches (config token "block=") */
+ST_API int NS(TrackJob_block_size)( const NS(TrackJobBase)* job );
 
 /* Output-buffer sizing used by the Python binding before it builds its
  * own output buffer (output/output_buffer.c:44-131 + 288-363):
@@ -247,6 +338,19 @@ ST_API NS(arch_size_t) NS(TrackJob_min_stored_buffer_id)( const NS(TrackJobBase)
 ST_API NS(arch_size_t) NS(TrackJob_max_stored_buffer_id)( const NS(TrackJobBase)* job );
 ST_API bool NS(TrackJob_owns_stored_buffer)( const NS(TrackJobBase)* job, NS(arch_size_t) buffer_id );
 ST_API NS(Buffer)* NS(TrackJob_stored_buffer)( NS(TrackJobBase)* job, NS(arch_size_t) buffer_id );
+ST_API NS(Buffer) const* NS(TrackJob_const_stored_buffer)( const NS(TrackJobBase)* job,
+    NS(arch_size_t) buffer_id );
+ST_API NS(arch_size_t) NS(TrackJob_stored_buffers_capacity)( const NS(TrackJobBase)* job );
+ST_API NS(arch_status_t) NS(TrackJob_reserve_stored_buffers_capacity)( NS(TrackJobBase)* job,
+    NS(arch_size_t) capacity );
+/* buffer ids 1 (particles), 2 (beam elements), 3 (output), 1024 + k
+ * (stored buffers); no id names raw memory in this library */
+ST_API NS(Buffer)* NS(TrackJob_buffer_by_buffer_id)( NS(TrackJobBase)* job, NS(arch_size_t) buffer_id );
+ST_API NS(Buffer) const* NS(TrackJob_const_buffer_by_buffer_id)( const NS(TrackJobBase)* job,
+    NS(arch_size_t) buffer_id );
+ST_API bool NS(TrackJob_is_buffer_by_buffer_id)( const NS(TrackJobBase)* job, NS(arch_size_t) buffer_id );
+ST_API bool NS(TrackJob_is_raw_memory_by_buffer_id)( const NS(TrackJobBase)* job,
+    NS(arch_size_t) buffer_id );
 ST_API NS(arch_status_t) NS(TrackJob_push_stored_buffer)( NS(TrackJobBase)* job,
     NS(arch_size_t) buffer_id );
 ST_API NS(arch_status_t) NS(TrackJob_collect_stored_buffer)( NS(TrackJobBase)* job,
@@ -265,6 +369,71 @@ ST_API NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item_detailed)(
     NS(buffer_size_t) dest_elem_index, NS(buffer_size_t) dest_pointer_offset,
     NS(object_type_id_t) src_type_id, NS(buffer_size_t) src_buffer_id,
     NS(buffer_size_t) src_elem_index, NS(buffer_size_t) src_pointer_offset );
+/* the reference header also declares this name (track_job.h:631) */
+ST_API NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item_detail)( NS(TrackJobBase)* job,
+    NS(object_type_id_t) dest_type_id, NS(arch_size_t) dest_buffer_id,
+    NS(arch_size_t) dest_elem_index, NS(arch_size_t) dest_pointer_offset,
+    NS(object_type_id_t) src_type_id, NS(arch_size_t) src_buffer_id,
+    NS(arch_size_t) src_elem_index, NS(arch_size_t) src_pointer_offset );
+ST_API NS(arch_status_t) NS(TrackJob_remove_assign_address_item)( NS(TrackJobBase)* job,
+    NS(AssignAddressItem)* item_to_remove );
+ST_API NS(arch_status_t) NS(TrackJob_remove_assign_address_item_by_key_and_index)(
+    NS(TrackJobBase)* job, const NS(TrackJobDestSrcBufferIds)* key, NS(buffer_size_t) index );
+ST_API bool NS(TrackJob_has_assign_address_item)( const NS(TrackJobBase)* job,
+    const NS(AssignAddressItem)* item );
+ST_API bool NS(TrackJob_has_assign_item_by_index)( const NS(TrackJobBase)* job,
+    NS(buffer_size_t) dest_buffer_id, NS(buffer_size_t) src_buffer_id, NS(buffer_size_t) index );
+ST_API bool NS(TrackJob_has_assign_address_item_detailed)( const NS(TrackJobBase)* job,
+    NS(object_type_id_t) dest_type_id, NS(buffer_size_t) dest_buffer_id,
+    NS(buffer_size_t) dest_elem_index, NS(buffer_size_t) dest_pointer_offset,
+    NS(object_type_id_t) src_type_id, NS(buffer_size_t) src_buffer_id,
+    NS(buffer_size_t) src_elem_index, NS(buffer_size_t) src_pointer_offset );
+/* index among the items of its (dest, src) pair; the pair's item count
+ * when there is no such item */
+ST_API NS(buffer_size_t) NS(TrackJob_index_of_assign_address_item_detailed)(
+    const NS(TrackJobBase)* job,
+    NS(object_type_id_t) dest_type_id, NS(buffer_size_t) dest_buffer_id,
+    NS(buffer_size_t) dest_elem_index, NS(buffer_size_t) dest_pointer_offset,
+    NS(object_type_id_t) src_type_id, NS(buffer_size_t) src_buffer_id,
+    NS(buffer_size_t) src_elem_index, NS(buffer_size_t) src_pointer_offset );
+ST_API NS(buffer_size_t) NS(TrackJob_index_of_assign_address_item)( const NS(TrackJobBase)* job,
+    const NS(AssignAddressItem)* item );
+ST_API NS(AssignAddressItem) const* NS(TrackJob_ptr_assign_address_item)( const NS(TrackJobBase)* job,
+    const NS(AssignAddressItem)* item );
+ST_API NS(AssignAddressItem) const* NS(TrackJob_ptr_assign_address_item_detailed)(
+    const NS(TrackJobBase)* job,
+    NS(object_type_id_t) dest_type_id, NS(buffer_size_t) dest_buffer_id,
+    NS(buffer_size_t) dest_elem_index, NS(buffer_size_t) dest_pointer_offset,
+    NS(object_type_id_t) src_type_id, NS(buffer_size_t) src_buffer_id,
+    NS(buffer_size_t) src_elem_index, NS(buffer_size_t) src_pointer_offset );
+ST_API NS(AssignAddressItem) const* NS(TrackJob_ptr_assign_address_item_by_index)(
+    const NS(TrackJobBase)* job, NS(buffer_size_t) dest_buffer_id,
+    NS(buffer_size_t) src_buffer_id, NS(buffer_size_t) index );
+/* distinct (dest, src) pairs among the items, in the order first added */
+ST_API NS(buffer_size_t) NS(TrackJob_num_distinct_available_assign_address_items_dest_src_pairs)(
+    const NS(TrackJobBase)* job );
+ST_API NS(arch_size_t) NS(TrackJob_num_distinct_assign_items_dest_src_pairs)( const NS(TrackJobBase)* job );
+ST_API NS(buffer_size_t) NS(TrackJob_available_assign_address_items_dest_src_pairs)(
+    const NS(TrackJobBase)* job, NS(buffer_size_t) max_num_pairs,
+    NS(TrackJobDestSrcBufferIds)* pairs_begin );
+ST_API NS(arch_size_t) NS(TrackJob_assign_items_dest_src_pairs)( const NS(TrackJobBase)* job,
+    NS(arch_size_t) max_num_pairs, NS(AssignAddressItemDestSrcPair)* pairs_begin );
+ST_API NS(TrackJobDestSrcBufferIds) const* NS(TrackJob_assign_item_dest_src_begin)(
+    const NS(TrackJobBase)* job );
+ST_API NS(TrackJobDestSrcBufferIds) const* NS(TrackJob_assign_item_dest_src_end)(
+    const NS(TrackJobBase)* job );
+/* the items of one (dest, src) pair as CObjects (type 65536, one
+ * NS(AssignAddressItem) each) of a buffer the job rebuilds at each call;
+ * valid until the next call or change of the items */
+ST_API NS(Object) const* NS(TrackJob_assign_items_begin)( const NS(TrackJobBase)* job,
+    NS(buffer_size_t) dest_buffer_id, NS(buffer_size_t) src_buffer_id );
+ST_API NS(Object) const* NS(TrackJob_assign_items_end)( const NS(TrackJobBase)* job,
+    NS(buffer_size_t) dest_buffer_id, NS(buffer_size_t) src_buffer_id );
+/* the items of one (dest, src) pair lie next to each other: [begin, end) */
+ST_API NS(AssignAddressItem) const* NS(TrackJob_assign_items_dest_src_begin)(
+    const NS(TrackJobBase)* job, NS(arch_size_t) dest_buffer_id, NS(arch_size_t) src_buffer_id );
+ST_API NS(AssignAddressItem) const* NS(TrackJob_assign_items_dest_src_end)(
+    const NS(TrackJobBase)* job, NS(arch_size_t) dest_buffer_id, NS(arch_size_t) src_buffer_id );
 ST_API bool NS(TrackJob_has_assign_items)( const NS(TrackJobBase)* job,
     NS(arch_size_t) dest_buffer_id, NS(arch_size_t) src_buffer_id );
 ST_API NS(arch_size_t) NS(TrackJob_num_assign_items)( const NS(TrackJobBase)* job,

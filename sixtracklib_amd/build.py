@@ -39,7 +39,7 @@ def _jobs():
         j.append(([CC] + C_COMMON + ["-c", os.path.join(CSRC, src), "-o",
                                       os.path.join(BUILD, src + ".o")],
                   os.path.join(BUILD, src + ".o")))
-    for src in ("lattice_pack.cpp", "track_job.cpp", "batch.cpp", "adjoint.cpp"):
+    for src in ("lattice_pack.cpp", "track_job.cpp", "track_job_new.cpp", "batch.cpp", "adjoint.cpp"):
         if not os.path.exists(os.path.join(CSRC, src)):
             continue
         j.append(([HIPCC] + HIP_COMMON + ["-ffp-contract=off", "-x", "hip", "-c",

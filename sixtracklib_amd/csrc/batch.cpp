@@ -97,13 +97,15 @@ struct NS(TrackBatch)
             if( nw < 0 ) return fail( "lattice could not be packed" );
             std::vector< uint32_t > offs( ne + 1 );
             std::vector< uint32_t > mons( nm > 0 ? nm : 1 );
-            packed[ s ].assign( nw + 32, 0u );        // zero padding read ahead by the kernels
+            packed[ s ].assign( nw + ST_LAT_PAD_WORDS, 0u );   // zero padding read ahead by the kernels
             if( NS(Lattice_pack)( eb, packed[ s ].data(), nw, offs.data(), &ne, mons.data(), &nm ) != nw )
                 return fail( "lattice could not be packed" );
             for( int64_t i = 0; i < ne; ++i )
                 if( !ST_OP_IS_LEAN( ST_REC_OP( packed[ s ][ offs[ i ] ] ) ) )
                     return fail( "batched tracking needs lean lattices (no monitors, beam-beam, "
                                  "space charge or RF multipoles)" );
+            for( int64_t i = 0; i < ne; ++i )
+                run.max_record_words = std::max< int64_t >( run.max_record_words, offs[ i + 1 ] - offs[ i ] );
             n_elem[ s ] = ne;
             lat_off[ s ] = total_lat;
             total_lat += packed[ s ].size();
@@ -228,24 +230,39 @@ struct NS(TrackBatch)
         return *h_status;
     }
 
-    // Candidate (block, waves_per_cu, queues) timed on trial_turns turns of
-    // the batch itself, the three fastest a second time; the particle images
-    // are saved first and restored after every trial, so the batch state is
-    // unchanged.  One particle per lane, at 4 and 8 resident waves per SIMD
-    // (its register budget allows 8).
-    int autotune( int64_t trial_turns )
+    // Candidate configurations timed on trial_turns turns of the batch
+    // itself, the three fastest a second time.  The trials start after
+    // skip_turns turns tracked with the current configuration (default 10
+    // trials' worth), so that they see the batch after its early losses,
+    // not the first turns with every particle alive.  The particle images
+    // are saved first and restored at the end (and the advanced state before
+    // every trial), so the batch state is unchanged.  Candidates:
+    //   one particle per lane, scalar-load lattice: block 64/128/256, 4 or 8
+    //     resident waves per SIMD, one queue or one per XCD;
+    //   two particles per lane (128-particle tiles, 5 waves per SIMD): block
+    //     256, both queue layouts;
+    //   lattice staged through LDS (256-lane workgroups): 4 or 8 waves per
+    //     SIMD, both queue layouts.
+    int autotune( int64_t trial_turns, int64_t skip_turns )
     {
         if( trial_turns <= 0 ) trial_turns = 2;
-        unsigned char* backup = nullptr;
-        if( !hip_ok( hipMalloc( &backup, std::max< size_t >( total_img, 256 ) ), "hipMalloc backup" ) )
-            return -1;
+        if( skip_turns < 0 ) skip_turns = 10 * trial_turns;
+        size_t const bytes = std::max< size_t >( total_img, 256 );
+        unsigned char* backup = nullptr;     // the state on entry
+        unsigned char* window = nullptr;     // the state the trials start from
+        if( !hip_ok( hipMalloc( &backup, bytes ), "hipMalloc backup" ) ) return -1;
+        if( !hip_ok( hipMalloc( &window, bytes ), "hipMalloc backup" ) )
+        { ( void )hipFree( backup ); return -1; }
+        Config const base = run.cfg;
+        int64_t const t_entry = min_turn;
         bool ok = hip_ok( hipMemcpyAsync( backup, d_img, total_img, hipMemcpyDeviceToDevice, stream ),
                           "backup" ) && hip_ok( hipStreamSynchronize( stream ), "sync" );
-        Config const base = run.cfg;
-        int64_t const t0 = min_turn;
-        // one trial per candidate, then the three fastest timed once more and
-        // ranked by the better of their two trials: a single short trial
-        // can rank a slower candidate first
+        if( ok && skip_turns > 0 )
+            ok = ( run.cfg.balance ? track_until_balanced( t_entry + skip_turns )
+                                   : track_until( t_entry + skip_turns ) ) >= -1 && last_ms >= 0.0f;
+        int64_t const t0 = ok ? std::max( min_turn, t_entry ) : t_entry;
+        ok = ok && hip_ok( hipMemcpyAsync( window, d_img, total_img, hipMemcpyDeviceToDevice, stream ),
+                           "backup" ) && hip_ok( hipStreamSynchronize( stream ), "sync" );
         auto trial = [ & ]( Config const& c, float& ms ) -> bool
         {
             run.cfg = c;
@@ -254,32 +271,45 @@ struct NS(TrackBatch)
             int32_t const st = run.cfg.balance ? track_until_balanced( t0 + trial_turns )
                                                : track_until( t0 + trial_turns, trial_turns );
             ms = last_ms;
-            // the images go back to the saved state whatever the trial gave
+            // the images go back to the window's start whatever the trial gave
             bool const restored =
-                hip_ok( hipMemcpyAsync( d_img, backup, total_img, hipMemcpyDeviceToDevice, stream ),
+                hip_ok( hipMemcpyAsync( d_img, window, total_img, hipMemcpyDeviceToDevice, stream ),
                         "restore" ) &&
                 hip_ok( hipStreamSynchronize( stream ), "sync" );
             return restored && st >= -1 && ms >= 0.0f;
         };
-        // One particle per lane only: two per lane (128-particle tiles) won
-        // short trials, which start with every particle alive, but lost
-        // every full run with losses; it stays available as ppl=2.
+        std::vector< Config > cand;
+        for( int b : { 64, 128, 256 } )
+            for( int w : { 16, 32 } )
+                for( int q : { 1, 8 } )
+                {
+                    Config c = base;
+                    c.ppl = 1; c.staging = 0; c.block = b; c.waves_per_cu = w; c.queues = q;
+                    cand.push_back( c );
+                }
+        for( int q : { 1, 8 } )
+        {
+            Config c = base;
+            c.ppl = 2; c.staging = 0; c.block = 256; c.waves_per_cu = 20; c.queues = q;
+            cand.push_back( c );
+        }
+        if( run.max_record_words + 2 <= ( int64_t )ST_LAT_STAGE_WORDS )
+            for( int w : { 16, 32 } )
+                for( int q : { 1, 8 } )
+                {
+                    Config c = base;
+                    c.ppl = 1; c.staging = 1; c.block = 256; c.waves_per_cu = w; c.queues = q;
+                    cand.push_back( c );
+                }
         std::vector< std::pair< float, Config > > timed;
-        int const blocks[] = { 64, 128, 256 };
-        int const waves[] = { 16, 32 };
-        int const queues[] = { 1, 8 };
-        for( int b : blocks )
-            for( int w : waves )
-                for( int q : queues )
-                    {
-                        if( !ok ) break;
-                        Config c = base;
-                        c.ppl = 1; c.block = b; c.waves_per_cu = w; c.queues = q;
-                        c.graph = false;
-                        float ms = -1.0f;
-                        ok = trial( c, ms );
-                        if( ok ) timed.emplace_back( ms, c );
-                    }
+        for( Config c : cand )
+        {
+            if( !ok ) break;
+            c.graph = false;
+            float ms = -1.0f;
+            ok = trial( c, ms );
+            if( ok ) timed.emplace_back( ms, c );
+        }
         std::stable_sort( timed.begin(), timed.end(),
                           []( auto const& x, auto const& y ) { return x.first < y.first; } );
         Config best = base;
@@ -291,12 +321,18 @@ struct NS(TrackBatch)
             float const m = std::min( ms, timed[ r ].first );
             if( ok && ( best_ms < 0.0f || m < best_ms ) ) { best_ms = m; best = timed[ r ].second; }
         }
+        // back to the state on entry
+        bool const restored =
+            hip_ok( hipMemcpyAsync( d_img, backup, total_img, hipMemcpyDeviceToDevice, stream ),
+                    "restore" ) &&
+            hip_ok( hipStreamSynchronize( stream ), "sync" );
         ( void )hipFree( backup );
-        run.cfg = best;
+        ( void )hipFree( window );
+        run.cfg = ok ? best : base;
         run.cfg.graph = base.graph;
-        min_turn = t0;
+        min_turn = t_entry;
         if( !run.build_queues( stream ) ) return -1;
-        return ok ? 0 : -1;
+        return ( ok && restored ) ? 0 : -1;
     }
 };
 
@@ -355,7 +391,12 @@ int64_t NS(TrackBatch_get_config)( const NS(TrackBatch)* batch, char* out, int64
 
 int NS(TrackBatch_autotune)( NS(TrackBatch)* batch, int64_t trial_turns )
 {
-    return batch ? batch->autotune( trial_turns ) : -1;
+    return batch ? batch->autotune( trial_turns, -1 ) : -1;
+}
+
+int NS(TrackBatch_autotune_window)( NS(TrackBatch)* batch, int64_t skip_turns, int64_t trial_turns )
+{
+    return batch ? batch->autotune( trial_turns, skip_turns < 0 ? 0 : skip_turns ) : -1;
 }
 
 int64_t NS(TrackBatch_num_seeds)( const NS(TrackBatch)* batch ) { return batch ? batch->n_seeds() : -1; }

@@ -21,6 +21,7 @@
 
 #include "track_device.h"
 #include "track_lean.h"
+#include "track_lean_lds.h"
 #include "batch.h"
 
 #ifndef ST_KSUFFIX
@@ -288,6 +289,43 @@ KN( st_k_batch_track_heavy )( st_batch_sched_t s, int64_t until_turn, int64_t ma
     batch_track_tiles< true >( s, until_turn, max_turns, status );
 }
 
+// Lattice staged through LDS (track_lean_lds.h): a workgroup of 256 lanes
+// takes one 256-particle tile of one particle set at a time and walks the
+// lattice with its four waves together.  16 KB of LDS per workgroup, 8
+// workgroups per CU.
+__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
+KN( st_k_batch_track_lds )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
+                            int32_t* status )
+{
+    __shared__ uint64_t stage[ ST_LAT_STAGE_WORDS ];
+    __shared__ int next_tile;
+    ci32_t* const qbegin = ( ci32_t* )s.queue_begin;
+    int const nq = s.n_queues;
+    int const q0 = ( int )( blockIdx.x % ( unsigned )nq );
+    bool const one_tile = s.split != nullptr;
+    for( int qi = 0; qi < nq; ++qi )
+    {
+        int const q = ( q0 + qi ) % nq;
+        int const qb = qbegin[ q ], qe = qbegin[ q + 1 ];
+        for( ;; )
+        {
+            if( threadIdx.x == 0 ) next_tile = atomicAdd( &s.queue_next[ q ], 1 );
+            __syncthreads();
+            int const t = next_tile;
+            __syncthreads();                  // read by all before the next write
+            int seed = 0, tile = 0;
+            int32_t count = 0;
+            const int32_t* list = nullptr;
+            if( !find_tile< 256 >( s, qb, qe, t, seed, list, count, tile ) ) break;  // queue drained
+            SeedView const S = load_seed( s.seeds, seed );
+            track_lean_block( S.P, list, count, ( int64_t )tile * 256, S.lat, S.n_elem, until_turn,
+                              max_turns, status,
+                              ( __attribute__( ( address_space( 3 ) ) ) uint64_t* )stage );
+            if( one_tile ) return;
+        }
+    }
+}
+
 // Occupancy target of the two-particles-per-lane track kernel: its two
 // state sets fit 96 VGPRs, 5 waves per SIMD.
 #ifndef ST_LEAN2_WAVES_PER_EU
@@ -360,12 +398,15 @@ extern "C" hipError_t KN( st_launch_batch_select_split )( const st_batch_sched_t
 
 extern "C" hipError_t KN( st_launch_batch_track )( const st_batch_sched_t* s,
     int64_t until_turn, int64_t max_turns, int32_t* status, int block, int grid,
-    int particles_per_lane, int heavy, hipStream_t stream )
+    int particles_per_lane, int heavy, int staging, hipStream_t stream )
 {
     if( s->n_seeds <= 0 || s->n_queues <= 0 ) return hipSuccess;
     if( block <= 0 || block > 256 || ( block % 64 ) != 0 ) block = 256;
     if( grid <= 0 ) grid = 1;
-    if( heavy )
+    if( staging && !heavy )
+        hipLaunchKernelGGL( KN( st_k_batch_track_lds ), dim3( ( unsigned )grid ), dim3( 256 ), 0,
+                            stream, *s, until_turn, max_turns, status );
+    else if( heavy )
         hipLaunchKernelGGL( KN( st_k_batch_track_heavy ), dim3( ( unsigned )grid ),
                             dim3( ( unsigned )block ), 0, stream, *s, until_turn, max_turns, status );
     else if( particles_per_lane == 2 )

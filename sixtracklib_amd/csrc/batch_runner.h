@@ -64,14 +64,18 @@ struct Config
                                  // (set by the owner, one particle per lane)
     bool balance = false;        // run_balanced() available (split lists)
     bool wave_prio = true;       // waves with more turns left issue first
+    int staging = 0;             // lattice read through scalar loads (0) or staged
+                                 // through LDS by 256-lane workgroups (1; lean
+                                 // lattices, one particle per lane)
 
     std::string str() const
     {
-        char b[ 192 ];
+        char b[ 224 ];
         std::snprintf( b, sizeof( b ), "fma=%d block=%d waves_per_cu=%d queues=%d "
-                       "turns_per_launch=%lld graph=%d ppl=%d wave_prio=%d%s%s", fma ? 1 : 0, block,
-                       waves_per_cu, queues, ( long long )turns_per_launch, graph ? 1 : 0, ppl,
-                       wave_prio ? 1 : 0, heavy ? " heavy" : "", balance ? " balance" : "" );
+                       "turns_per_launch=%lld graph=%d ppl=%d wave_prio=%d staging=%s%s%s",
+                       fma ? 1 : 0, block, waves_per_cu, queues, ( long long )turns_per_launch,
+                       graph ? 1 : 0, ppl, wave_prio ? 1 : 0, staging ? "lds" : "scalar",
+                       heavy ? " heavy" : "", balance ? " balance" : "" );
         return b;
     }
 
@@ -104,8 +108,20 @@ struct Config
             if( iv != 1 && iv != 2 ) return fail( "ppl (particles per lane) must be 1 or 2" );
             ppl = ( int )iv;
         }
+        else if( k == "staging" )
+        {
+            if( v == "scalar" || v == "0" ) staging = 0;
+            else if( v == "lds" || v == "1" ) staging = 1;
+            else return fail( "staging must be scalar or lds" );
+        }
         return true;
     }
+
+    // the LDS-staged kernel runs lean lattices with one particle per lane
+    bool lds() const { return staging != 0 && !heavy; }
+    // particles per tile and tiles a workgroup tracks at once
+    int64_t tile_particles() const { return lds() ? 256 : 64 * ppl; }
+    int tiles_per_block() const { return lds() ? 1 : block / 64; }
 
     bool parse( const char* cfg )
     {
@@ -387,13 +403,24 @@ struct BatchRunner
         return k >= 1.0e9 ? ( int64_t )1000000000 : ( k < 1.0 ? 1 : ( int64_t )k );
     }
 
-    int64_t waves() const { return ( int64_t )grid() * ( cfg.block / 64 ); }
+    // tiles the persistent grid tracks at once (one per wave, or one per
+    // workgroup when the lattice is staged through LDS)
+    int64_t waves() const { return ( int64_t )grid() * cfg.tiles_per_block(); }
 
     int grid() const
     {
-        int const waves_per_block = cfg.block / 64;
+        int const waves_per_block = ( cfg.lds() ? 256 : cfg.block ) / 64;
         int const g = ( n_cu > 0 ? n_cu : 256 ) * cfg.waves_per_cu / waves_per_block;
         return g < cfg.queues ? cfg.queues : g;
+    }
+
+    // the longest record (words) of the lattices; the LDS-staged kernel
+    // needs every record, with the next one's header, in one stage
+    int64_t max_record_words = 0;
+    bool staging_ok() const
+    {
+        return !cfg.lds() || max_record_words + 2 <= ( int64_t )ST_LAT_STAGE_WORDS ||
+               fail( "a lattice record is longer than the LDS stage (staging=lds)" );
     }
 
     bool enqueue_reset_select( int64_t until_turn, hipStream_t stream ) const
@@ -415,9 +442,9 @@ struct BatchRunner
             if( !enqueue_reset_select( until_turn, stream ) ) return false;
             hipError_t const e = cfg.fma
                 ? st_launch_batch_track_fma( &s, until_turn, k, d_status, cfg.block, grid(), cfg.ppl,
-                                             cfg.heavy ? 1 : 0, stream )
+                                             cfg.heavy ? 1 : 0, cfg.lds() ? 1 : 0, stream )
                 : st_launch_batch_track_exact( &s, until_turn, k, d_status, cfg.block, grid(), cfg.ppl,
-                                               cfg.heavy ? 1 : 0, stream );
+                                               cfg.heavy ? 1 : 0, cfg.lds() ? 1 : 0, stream );
             if( !hip_ok( e, "launch batch track" ) ) return false;
         }
         return true;
@@ -456,7 +483,7 @@ struct BatchRunner
         r = std::max< int64_t >( r, 1 );
         int64_t const W = std::max< int64_t >( waves(), 1 );
         if( tiles <= W ) return std::min( chunk_turns( n_active ), r );
-        int64_t const kmax = std::min( chunk_turns( W * 64 * cfg.ppl ), r );
+        int64_t const kmax = std::min( chunk_turns( W * cfg.tile_particles() ), r );
         int64_t const vmin = ( r + kmax - 1 ) / kmax;
         int64_t best_k = kmax, best_cost = INT64_MAX;
         for( int64_t v = vmin; v <= std::min( vmin + 32, r ); ++v )
@@ -489,7 +516,7 @@ struct BatchRunner
     // defers tiles to the next launch.
     int grid_for_tiles( int64_t tiles ) const
     {
-        int64_t const per_block = cfg.block / 64;
+        int64_t const per_block = cfg.tiles_per_block();
         int64_t const g = ( tiles + per_block - 1 ) / per_block;
         return ( int )std::max< int64_t >( cfg.queues, std::min< int64_t >( grid(), g ) );
     }
@@ -500,9 +527,9 @@ struct BatchRunner
         st_batch_sched_t const s = sched_split();
         hipError_t const e = cfg.fma
             ? st_launch_batch_track_fma( &s, until_turn, k, d_status, cfg.block, g, cfg.ppl,
-                                         cfg.heavy ? 1 : 0, stream )
+                                         cfg.heavy ? 1 : 0, cfg.lds() ? 1 : 0, stream )
             : st_launch_batch_track_exact( &s, until_turn, k, d_status, cfg.block, g, cfg.ppl,
-                                           cfg.heavy ? 1 : 0, stream );
+                                           cfg.heavy ? 1 : 0, cfg.lds() ? 1 : 0, stream );
         return hip_ok( e, "launch batch track" );
     }
 
@@ -516,7 +543,7 @@ struct BatchRunner
         n_active = 0; tiles = 0; min_turn = until_turn;
         if( S == 0 ) return true;
         if( d_split == nullptr ) return fail( "balanced scheduling was not set up" );
-        int64_t const tile = 64 * cfg.ppl;
+        int64_t const tile = cfg.tile_particles();
         if( !enqueue_select_split( until_turn, stream ) ||
             !hip_ok( hipMemcpyAsync( h_split, d_split, ( 2 * S + 1 ) * sizeof( int32_t ),
                                      hipMemcpyDeviceToHost, stream ), "counts D2H" ) ||
@@ -541,6 +568,7 @@ struct BatchRunner
     // fault into an error.
     bool run_balanced( int64_t until_turn, int32_t* d_status, hipStream_t stream )
     {
+        if( !staging_ok() ) return false;
         int64_t n_active = 0, min_turn = 0, tiles = 0, k = 0, limit = 0, launches = 0;
         int64_t const W = std::max< int64_t >( waves(), 1 );
         for( int64_t group = 0;; ++group )
@@ -570,6 +598,7 @@ struct BatchRunner
     bool run( int64_t until_turn, int64_t chunks, int64_t k, int32_t* d_status, hipStream_t stream )
     {
         if( chunks <= 0 ) return true;
+        if( !staging_ok() ) return false;
         if( !cfg.graph ) return enqueue_chunks( until_turn, chunks, k, d_status, stream );
         char kb[ 96 ];
         std::snprintf( kb, sizeof( kb ), " until=%lld chunks=%lld k=%lld status=%p",

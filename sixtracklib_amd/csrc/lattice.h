@@ -19,6 +19,14 @@
  *          word 2… payload
  * The stream ends with an ST_OP_END record (followed by zero padding).
  *
+ * Drift and DriftExact records carry, after their length, the path length
+ * s after the drift on a turn that starts at s = 0: the running sum of the
+ * drift lengths from the lattice start, added in element order as the
+ * reference adds them to s (track.h:629-660).  A particle that starts its
+ * turn at s = +0 has exactly that s after each drift, so the lean loop need
+ * not add the length to every particle (it still does for a turn that
+ * starts elsewhere).
+ *
  * Opcodes with bit 7 clear are the two maps that make up nearly all of a
  * ring (Drift, Multipole): the lean kernel's inner loop tests that one bit
  * to stay on them, and a second bit to tell them apart.  Every other map
@@ -34,10 +42,10 @@
 
 enum st_opcode_e
 {
-    ST_OP_DRIFT              = 0x01, /* [L]                                  */
+    ST_OP_DRIFT              = 0x01, /* [L, s after the drift]               */
     ST_OP_MULTIPOLE          = 0x02, /* aux: order | 0x8000 curvature;
                                         [bal 0..2n+1] (+ [hxl, hyl, length]) */
-    ST_OP_DRIFT_EXACT        = 0x81, /* [L]                                  */
+    ST_OP_DRIFT_EXACT        = 0x81, /* [L, s after the drift]               */
     ST_OP_CAVITY             = 0x82, /* [V, (pi/180)*lag, ((2pi)/c)*f]       */
     ST_OP_XYSHIFT            = 0x83, /* [-dx, -dy]                           */
     ST_OP_SROTATION          = 0x84, /* [cos, sin]                           */
@@ -72,6 +80,12 @@ enum st_opcode_e
 #define ST_OP_IS_LEAN( op ) ( ( op ) == ST_OP_DRIFT || ( op ) == ST_OP_MULTIPOLE || \
     ( ( op ) >= ST_OP_DRIFT_EXACT && ( op ) <= ST_OP_DIPEDGE ) || ( op ) == ST_OP_END )
 #define ST_MP_CURV_FLAG 0x8000u
+/* Lattice staging through LDS (track_lean_lds.h) copies the stream in
+ * pieces of ST_LAT_STAGE_WORDS words; every packed stream is followed by
+ * ST_LAT_PAD_WORDS zero words, so that such a piece starting at any record
+ * and the kernels' read-ahead of the next record stay inside it. */
+#define ST_LAT_STAGE_WORDS 2048
+#define ST_LAT_PAD_WORDS   ( 32 + ST_LAT_STAGE_WORDS )
 
 /* Device-side particle set: count + 17 FP64 + 4 int64 field pointers, the
  * NS(Particles) layout (sixtracklib/common/particles.h:38-67). */

@@ -58,6 +58,10 @@ struct Packer
     }
 
     uint64_t index = 0;                       // position of the object being packed
+    // path length after the drifts packed so far, summed from 0.0 in element
+    // order with the reference's additions (track.h:629-660: s += length per
+    // Drift / DriftExact, s reset to 0 at the turn's end)
+    double s_run = 0.0;
 
     void rec( unsigned op, unsigned aux, const std::vector< uint64_t >& payload )
     {
@@ -72,8 +76,16 @@ struct Packer
         if( !inside( o.begin, o.size ) ) return false;
         switch( o.type )
         {
-        case ST_OBJECT_TYPE_DRIFT: rec( ST_OP_DRIFT, 0, { e[ 0 ] } ); break;
-        case ST_OBJECT_TYPE_DRIFT_EXACT: rec( ST_OP_DRIFT_EXACT, 0, { e[ 0 ] } ); break;
+        case ST_OBJECT_TYPE_DRIFT:
+        case ST_OBJECT_TYPE_DRIFT_EXACT:
+        {
+            double length;
+            std::memcpy( &length, &e[ 0 ], 8 );
+            s_run += length;
+            rec( o.type == ST_OBJECT_TYPE_DRIFT ? ST_OP_DRIFT : ST_OP_DRIFT_EXACT, 0,
+                 { e[ 0 ], as_u64( s_run ) } );
+            break;
+        }
         case ST_OBJECT_TYPE_MULTIPOLE:
         {
             int64_t order; std::memcpy( &order, &e[ 0 ], 8 );

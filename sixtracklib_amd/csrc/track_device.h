@@ -54,35 +54,39 @@ struct LaneConst
     double q0, mass0, beta0, p0c, chi, charge_ratio;
 };
 
-// be_drift/track.h:45-67.  dzeta = rvv - (1 + (xp^2 + yp^2)/2) cancels:
-// 1 + t/2 rounds t/2 (about 1e-10) to the spacing of doubles near 1, so
-// the zeta gained per drift carries a relative rounding error of order
-// 1e-7.  The FMA build keeps exactly those roundings (t from two unfused
-// products and a sum, 1 + t/2 rounded once, as fma(t, 0.5, 1) does since
-// t * 0.5 is exact) and fuses only the updates of x, y and zeta, which
-// move each result by at most an ulp of its own size.
-ST_DEV void drift( Lane& p, double const length )
+// be_drift/track.h:45-67 without the path length s (drift() adds it).
+// dzeta = rvv - (1 + (xp^2 + yp^2)/2) cancels: 1 + t/2 rounds t/2 (about
+// 1e-10) to the spacing of doubles near 1, so the zeta gained per drift
+// carries a relative rounding error of order 1e-7.  The FMA build keeps
+// that rounding (1 + t/2 rounded once, as fma(t, 0.5, 1) does since t * 0.5
+// is exact) and fuses the updates of x, y and zeta, which move each result
+// by at most an ulp of its own size; t = xp^2 + yp^2 is one fused
+// multiply-add there, an ulp of t (~1e-26) from the reference's, which
+// changes 1 + t/2 only when that sum lies within 1e-26 of a rounding
+// boundary.
+ST_DEV void drift_xz( Lane& p, double const length )
 {
     double const xp = p.px * p.rpp;
     double const yp = p.py * p.rpp;
 #if defined( ST_FMA_BUILD )
-    double t;
-    {
-#pragma clang fp contract( off )
-        t = xp * xp + yp * yp;
-    }
+    double const t = fma( xp, xp, yp * yp );
     double const dzeta = p.rvv - fma( t, 0.5, 1.0 );
     p.x = fma( xp, length, p.x );
     p.y = fma( yp, length, p.y );
-    p.s += length;
     p.zeta = fma( length, dzeta, p.zeta );
 #else
     double const dzeta = p.rvv - ( 1.0 + ( xp * xp + yp * yp ) / 2.0 );
     p.x += xp * length;
     p.y += yp * length;
-    p.s += length;
     p.zeta += length * dzeta;
 #endif
+}
+
+// be_drift/track.h:45-67
+ST_DEV void drift( Lane& p, double const length )
+{
+    drift_xz( p, length );
+    p.s += length;
 }
 
 // be_drift/track.h:71-97

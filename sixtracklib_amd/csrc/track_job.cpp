@@ -226,6 +226,8 @@ struct NS(TrackJobBase)
     int ppl = 1;                         // particles per lane of the lean until path
     bool balance = true;                 // one tile per wave and launch, furthest behind first
     bool wave_prio = true;               // waves with more turns left issue first
+    int staging = 0;                     // lean until path: lattice via scalar loads or LDS
+    int64_t max_rec_words = 0;           // longest record of the packed lattice
     int waves_per_cu = 0;                // resident waves per CU (0: the kernel's occupancy)
     st_batch::BatchRunner runner;        // scheduling of the lean until path
     bool runner_ready = false;
@@ -239,11 +241,27 @@ struct NS(TrackJobBase)
     float last_kernel_ms = 0.0f;
     std::vector< st_dev_particles_t > views;
     st_dev_particles_t ebe_view{};
+    // introspection arrays (track_job.h:360-470): particles per set,
+    // element indices of the beam monitors, the element-by-element
+    // configuration as the host sees it
+    std::vector< uint64_t > set_sizes_u64, monitor_indices_u64;
+    st_ebe_config_t ebe_cfg_host{};
+    // particle addresses (track_job.h:206-270): one NS(ParticlesAddr) per
+    // Particles object of the particle buffer, holding the device
+    // addresses of the job's copy of its 21 field arrays
+    NS(Buffer)* paddr_buf = nullptr;
+    bool has_paddr = false;
+    // distinct (destination, source) pairs of the assignment items, and
+    // the items of one pair as CObjects (rebuilt on request)
+    std::vector< NS(AssignAddressItemDestSrcPair) > pair_cache;
+    NS(Buffer)* item_objs = nullptr;
+    uint64_t items_version = 0, item_objs_version = ~0ull, item_objs_dest = 0, item_objs_src = 0;
 
     ~NS(TrackJobBase)()
     {
         release();
         for( Stored& b : stored ) if( b.owned && b.buf ) NS(Buffer_delete)( b.buf );
+        if( item_objs ) NS(Buffer_delete)( item_objs );
     }
 
     // -- stored buffers and address assignments (track_job_base.cpp:
@@ -317,6 +335,58 @@ struct NS(TrackJobBase)
         h_status = nullptr; stream = nullptr; ev_start = ev_stop = nullptr;
         if( owns_obuf && obuf ) NS(Buffer_delete)( obuf );
         obuf = nullptr; owns_obuf = false;
+        if( paddr_buf ) NS(Buffer_delete)( paddr_buf );
+        paddr_buf = nullptr; has_paddr = false;
+    }
+
+    // NS(ParticlesAddr_prepare_buffer_based_on_particles_buffer): one
+    // zeroed address record per Particles object of the particle buffer
+    bool prepare_paddr()
+    {
+        paddr_buf = NS(Buffer_new)( 0 );
+        if( paddr_buf == nullptr ) { g_err = "cannot allocate the particle-address buffer"; return false; }
+        uint64_t const n = NS(Buffer_get_num_of_objects)( pbuf );
+        for( uint64_t k = 0; k < n; ++k )
+        {
+            NS(ParticlesAddr) a{};
+            if( NS(Buffer_add_object)( paddr_buf, &a, sizeof( a ), ST_OBJECT_TYPE_PARTICLES_ADDR, 0,
+                                       nullptr, nullptr, nullptr ) == nullptr )
+            { g_err = "cannot fill the particle-address buffer"; return false; }
+        }
+        has_paddr = false;
+        return true;
+    }
+
+    NS(ParticlesAddr)* paddr( uint64_t k ) const
+    {
+        if( paddr_buf == nullptr || k >= NS(Buffer_get_num_of_objects)( paddr_buf ) ) return nullptr;
+        NS(Object)* o = NS(Buffer_get_object)( paddr_buf, k );
+        return o ? ( NS(ParticlesAddr)* )( uintptr_t )o->begin_addr : nullptr;
+    }
+
+    // the device addresses of every Particles object of the job's device
+    // copy of the particle buffer (the reference's kernel fetches them on
+    // the device; here they follow from the image's relocation)
+    bool fetch_paddr()
+    {
+        if( paddr_buf == nullptr || d_pbuf == nullptr ) return false;
+        uint64_t const n = NS(Buffer_get_num_of_objects)( pbuf );
+        for( uint64_t k = 0; k < n; ++k )
+        {
+            NS(ParticlesAddr)* a = paddr( k );
+            if( a == nullptr ) return false;
+            std::memset( a, 0, sizeof( *a ) );
+            if( NS(Particles_buffer_get_const_particles)( pbuf, k ) == nullptr ) continue;
+            st_dev_particles_t const v = device_view( pbuf, d_pbuf, k );
+            a->num_particles = v.n;
+            uint64_t* dst = &a->q0_addr;
+            double* const* fr = &v.q0;
+            for( int f = 0; f < 17; ++f ) dst[ f ] = ( uint64_t )( uintptr_t )fr[ f ];
+            int64_t* const* fi = &v.particle_id;
+            for( int f = 0; f < 4; ++f ) dst[ 17 + f ] = ( uint64_t )( uintptr_t )fi[ f ];
+        }
+        has_paddr = true;
+        return true;
     }
 
     // -- host <-> device images ------------------------------------------
@@ -373,7 +443,7 @@ struct NS(TrackJobBase)
         if( nw < 0 ) { g_err = "lattice could not be packed"; return false; }
         // zero padding: the kernels read the header one record ahead and
         // fetch multipole coefficients in batches of up to 16 words
-        lat_host.assign( nw + 32, 0u );
+        lat_host.assign( nw + ST_LAT_PAD_WORDS, 0u );
         rec_offsets.assign( n_elem_l + 1, 0u );
         monitor_fields.assign( n_mon, 0u );
         if( NS(Lattice_pack_with_tables)( eb, tables.data(), nt, lat_host.data(), nw,
@@ -382,10 +452,12 @@ struct NS(TrackJobBase)
         n_elem = n_elem_l;
         lat_words = nw;
         lat_full = false;
+        max_rec_words = 0;
         for( int64_t i = 0; i < n_elem; ++i )
         {
             unsigned const op = ST_REC_OP( lat_host[ rec_offsets[ i ] ] );
             if( !ST_OP_IS_LEAN( op ) ) lat_full = true;
+            max_rec_words = std::max< int64_t >( max_rec_words, rec_offsets[ i + 1 ] - rec_offsets[ i ] );
         }
         // point the monitors that have an output set at its device header
         if( plan.monitor_elems.size() > 0 && d_obuf != nullptr )
@@ -517,7 +589,10 @@ struct NS(TrackJobBase)
         views.clear();
         for( uint64_t k : sets ) views.push_back( device_view( pbuf, d_pbuf, k ) );
         if( plan.has_ebe ) ebe_view = device_view( obuf, d_obuf, ebe_offset );
-        return true;
+        set_sizes_u64.assign( set_sizes.begin(), set_sizes.end() );
+        monitor_indices_u64.assign( plan.monitor_elems.begin(), plan.monitor_elems.end() );
+        ebe_cfg_host = ebe_cfg;
+        return prepare_paddr();
     }
 
     // status word cleared before and read back after the launches; the two
@@ -566,6 +641,7 @@ struct NS(TrackJobBase)
         c.ppl = lat_full ? 1 : ppl;
         c.balance = balance;
         c.wave_prio = wave_prio;
+        c.staging = ( lat_full || c.ppl == 2 ) ? 0 : staging;
         // resident waves per CU: 8 per SIMD with one particle per lane, 5
         // with two (the kernels' register budgets)
         c.waves_per_cu = c.ppl == 2 ? 20 : 32;
@@ -578,6 +654,7 @@ struct NS(TrackJobBase)
     {
         if( runner_ready ) return true;
         runner.cfg = runner_config();
+        runner.max_record_words = max_rec_words;
         std::vector< st_batch_seed_t > seeds;
         for( const st_dev_particles_t& v : views )
         {
@@ -647,6 +724,7 @@ struct NS(TrackJobBase)
         if( compact && n_elem > 0 )
         {
             na_runner.cfg = runner_config();
+            na_runner.max_record_words = max_rec_words;
             ok = na_runner.init_borrowed( views.data(), d_lat, n_elem, S,
                                           base + ST_NOALLOC_STATUS_BYTES,
                                           ST_NOALLOC_SCRATCH_BYTES - ST_NOALLOC_STATUS_BYTES,
@@ -765,6 +843,7 @@ struct NS(TrackJobBase)
             else if( k == "ppl" ) ppl = ( iv == 2 ) ? 2 : 1;
             else if( k == "balance" ) balance = iv != 0;
             else if( k == "wave_prio" ) wave_prio = iv != 0;
+            else if( k == "staging" ) staging = ( v == "lds" || v == "1" ) ? 1 : 0;
             else if( k == "waves_per_cu" ) waves_per_cu = ( iv >= 1 && iv <= 32 ) ? ( int )iv : 0;
         }
     }
@@ -969,6 +1048,10 @@ NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* j ) { retur
 double NS(TrackJob_last_kernel_time_ms)( const NS(TrackJobBase)* j ) { return j ? ( double )j->last_kernel_ms : -1.0; }
 int NS(TrackJob_uses_fma)( const NS(TrackJobBase)* j ) { return ( j && j->fma ) ? 1 : 0; }
 int NS(TrackJob_uses_full_kernel)( const NS(TrackJobBase)* j ) { return ( j && j->lat_full ) ? 1 : 0; }
+int NS(TrackJob_block_size)( const NS(TrackJobBase)* j )
+{
+    return j ? ( ( j->block == 64 || j->block == 128 ) ? j->block : 256 ) : 0;
+}
 
 int64_t NS(TrackJob_required_output_sets)( NS(Buffer)* pbuf, NS(buffer_size_t) nsets,
     NS(buffer_size_t) const* idx, NS(Buffer)* ebuf, NS(buffer_size_t) ute,
@@ -1089,8 +1172,15 @@ NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item)( NS(TrackJobBase)* j
     if( j == nullptr || item == nullptr ) return nullptr;
     for( NS(AssignAddressItem)& it : j->assign_items )
         if( std::memcmp( &it, item, sizeof( it ) ) == 0 ) return &it;   // already there
-    j->assign_items.push_back( *item );
-    return &j->assign_items.back();
+    // after the last item of the same (dest, src) pair: a pair's items stay
+    // next to each other, in the order added
+    size_t pos = j->assign_items.size();
+    for( size_t k = 0; k < j->assign_items.size(); ++k )
+        if( j->assign_items[ k ].dest_buffer_id == item->dest_buffer_id &&
+            j->assign_items[ k ].src_buffer_id == item->src_buffer_id ) pos = k + 1;
+    j->assign_items.insert( j->assign_items.begin() + pos, *item );
+    ++j->items_version;
+    return &j->assign_items[ pos ];
 }
 
 NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item_detailed)( NS(TrackJobBase)* j,
@@ -1144,5 +1234,443 @@ NS(arch_status_t) NS(TrackJob_assign_addresses)( NS(TrackJobBase)* j,
         ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
 }
 
+
+// -- arch / device strings (track_job.h:123-135) ---------------------------
+int NS(TrackJob_extract_device_id_str)( char const* conf, char* out, NS(buffer_size_t) max_len )
+{
+    if( conf == nullptr ) return -1;
+    if( out != nullptr && max_len > 0u ) std::memset( out, 0, max_len );
+    std::string id;
+    std::string const c = conf;
+    size_t const at = c.find( "device=" );
+    if( at != std::string::npos && ( at == 0 || c[ at - 1 ] == ' ' || c[ at - 1 ] == ',' ||
+                                     c[ at - 1 ] == ';' ) )
+    {
+        size_t e = at + 7;
+        while( e < c.size() && std::isalnum( ( unsigned char )c[ e ] ) ) ++e;
+        id = c.substr( at + 7, e - at - 7 );
+    }
+    else
+    {
+        // a leading "P.D" (platform.device), the reference's device id form
+        size_t b = 0;
+        while( b < c.size() && std::isspace( ( unsigned char )c[ b ] ) ) ++b;
+        size_t e = b;
+        while( e < c.size() && std::isdigit( ( unsigned char )c[ e ] ) ) ++e;
+        if( e > b && e < c.size() && c[ e ] == '.' )
+        {
+            size_t f = e + 1;
+            while( f < c.size() && std::isdigit( ( unsigned char )c[ f ] ) ) ++f;
+            if( f > e + 1 ) id = c.substr( b, f - b );
+        }
+    }
+    if( id.empty() ) return 0;
+    if( out == nullptr || id.size() + 1u > max_len ) return -1;
+    std::memcpy( out, id.data(), id.size() );
+    return 0;
+}
+
+int NS(TrackJob_sanitize_arch_str)( char const* arch, char* out, NS(buffer_size_t) max_len )
+{
+    if( arch == nullptr || out == nullptr || max_len == 0u ) return -1;
+    std::string a;
+    for( char const* c = arch; *c; ++c )
+        if( !std::isspace( ( unsigned char )*c ) ) a.push_back( ( char )std::tolower( ( unsigned char )*c ) );
+    if( a.empty() || a.size() + 1u > max_len ) return -1;
+    std::memset( out, 0, max_len );
+    std::memcpy( out, a.data(), a.size() );
+    return 0;
+}
+
+int NS(TrackJob_sanitize_arch_str_inplace)( char* arch, NS(buffer_size_t) max_len )
+{
+    if( arch == nullptr || std::strlen( arch ) >= max_len ) return -1;
+    std::string const copy = arch;
+    return NS(TrackJob_sanitize_arch_str)( copy.c_str(), arch, max_len );
+}
+
+// -- particle addresses (track_job.h:206-290) ------------------------------
+void NS(TrackJob_collect_particles_addresses)( NS(TrackJobBase)* ) {}
+void NS(TrackJob_push_particles_addresses)( NS(TrackJobBase)* ) {}
+bool NS(TrackJob_can_fetch_particles_addr)( const NS(TrackJobBase)* j ) { return j && j->paddr_buf; }
+bool NS(TrackJob_has_particles_addr)( const NS(TrackJobBase)* j ) { return j && j->has_paddr; }
+
+NS(arch_status_t) NS(TrackJob_fetch_particle_addresses)( NS(TrackJobBase)* j )
+{
+    return ( j && j->fetch_paddr() ) ? ST_ARCH_STATUS_SUCCESS : ST_ARCH_STATUS_GENERAL_FAILURE;
+}
+NS(arch_status_t) NS(TrackJob_fetch_particles_addr)( NS(TrackJobBase)* j )
+{
+    return NS(TrackJob_fetch_particle_addresses)( j );
+}
+
+NS(arch_status_t) NS(TrackJob_clear_particle_addresses)( NS(TrackJobBase)* j, NS(arch_size_t) k )
+{
+    NS(ParticlesAddr)* a = j ? j->paddr( k ) : nullptr;
+    if( a == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    std::memset( a, 0, sizeof( *a ) );
+    j->has_paddr = false;
+    return ST_ARCH_STATUS_SUCCESS;
+}
+NS(arch_status_t) NS(TrackJob_clear_particles_addr)( NS(TrackJobBase)* j, NS(buffer_size_t) k )
+{
+    return NS(TrackJob_clear_particle_addresses)( j, k );
+}
+
+NS(arch_status_t) NS(TrackJob_clear_all_particle_addresses)( NS(TrackJobBase)* j )
+{
+    if( j == nullptr || j->paddr_buf == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    for( uint64_t k = 0; k < NS(Buffer_get_num_of_objects)( j->paddr_buf ); ++k )
+        std::memset( j->paddr( k ), 0, sizeof( NS(ParticlesAddr) ) );
+    j->has_paddr = false;
+    return ST_ARCH_STATUS_SUCCESS;
+}
+NS(arch_status_t) NS(TrackJob_clear_all_particles_addr)( NS(TrackJobBase)* j )
+{
+    return NS(TrackJob_clear_all_particle_addresses)( j );
+}
+
+NS(ParticlesAddr) const* NS(TrackJob_particle_addresses)( const NS(TrackJobBase)* j, NS(arch_size_t) k )
+{
+    return ( j && j->has_paddr ) ? j->paddr( k ) : nullptr;
+}
+NS(ParticlesAddr) const* NS(TrackJob_particles_addr)( const NS(TrackJobBase)* j, NS(buffer_size_t) k )
+{
+    return NS(TrackJob_particle_addresses)( j, k );
+}
+NS(Buffer)* NS(TrackJob_get_particles_addr_buffer)( NS(TrackJobBase)* j ) { return j ? j->paddr_buf : nullptr; }
+NS(Buffer) const* NS(TrackJob_get_const_particles_addr_buffer)( const NS(TrackJobBase)* j )
+{
+    return j ? j->paddr_buf : nullptr;
+}
+NS(Buffer) const* NS(TrackJob_particles_addr_buffer)( const NS(TrackJobBase)* j ) { return j ? j->paddr_buf : nullptr; }
+
+// -- reset of one set, set sizes, const buffers, monitors, ebe config -----
+bool NS(TrackJob_reset_particle_set)( NS(TrackJobBase)* job, NS(Buffer)* pbuf, NS(buffer_size_t) k,
+    NS(Buffer)* ebuf, NS(Buffer)* obuf )
+{
+    return NS(TrackJob_reset_detailed)( job, pbuf, 1u, &k, ebuf, obuf,
+                                        job ? job->until_turn_ebe : 0u );
+}
+
+NS(buffer_size_t) const* NS(TrackJob_get_num_particles_in_sets_begin)( const NS(TrackJobBase)* j )
+{
+    return j ? j->set_sizes_u64.data() : nullptr;
+}
+NS(buffer_size_t) const* NS(TrackJob_get_num_particles_in_sets_end)( const NS(TrackJobBase)* j )
+{
+    return j ? j->set_sizes_u64.data() + j->set_sizes_u64.size() : nullptr;
+}
+NS(buffer_size_t) NS(TrackJob_get_num_particles_in_set)( const NS(TrackJobBase)* j, NS(buffer_size_t) n )
+{
+    return ( j && n < j->set_sizes_u64.size() ) ? j->set_sizes_u64[ n ] : 0u;
+}
+
+NS(Buffer) const* NS(TrackJob_get_const_particles_buffer)( const NS(TrackJobBase)* j ) { return j ? j->pbuf : nullptr; }
+NS(Buffer) const* NS(TrackJob_get_const_beam_elements_buffer)( const NS(TrackJobBase)* j ) { return j ? j->ebuf : nullptr; }
+NS(Buffer) const* NS(TrackJob_get_const_output_buffer)( const NS(TrackJobBase)* j ) { return j ? j->obuf : nullptr; }
+
+bool NS(TrackJob_has_beam_monitors)( const NS(TrackJobBase)* j ) { return j && !j->monitor_indices_u64.empty(); }
+NS(buffer_size_t) const* NS(TrackJob_get_beam_monitor_indices_begin)( const NS(TrackJobBase)* j )
+{
+    return j ? j->monitor_indices_u64.data() : nullptr;
+}
+NS(buffer_size_t) const* NS(TrackJob_get_beam_monitor_indices_end)( const NS(TrackJobBase)* j )
+{
+    return j ? j->monitor_indices_u64.data() + j->monitor_indices_u64.size() : nullptr;
+}
+NS(buffer_size_t) NS(TrackJob_get_beam_monitor_index)( const NS(TrackJobBase)* j, NS(buffer_size_t) n )
+{
+    return ( j && n < j->monitor_indices_u64.size() ) ? j->monitor_indices_u64[ n ] : 0u;
+}
+
+static_assert( sizeof( NS(ElemByElemConfig) ) == sizeof( st_ebe_config_t ),
+               "NS(ElemByElemConfig) and the kernels' configuration share their layout" );
+
+bool NS(TrackJob_has_elem_by_elem_config)( const NS(TrackJobBase)* j ) { return j && j->plan.has_ebe; }
+NS(ElemByElemConfig) const* NS(TrackJob_get_elem_by_elem_config)( const NS(TrackJobBase)* j )
+{
+    return ( j && j->plan.has_ebe ) ? reinterpret_cast< NS(ElemByElemConfig) const* >( &j->ebe_cfg_host )
+                                    : nullptr;
+}
+bool NS(TrackJob_is_elem_by_elem_config_rolling)( const NS(TrackJobBase)* j )
+{
+    return j && j->plan.has_ebe && j->ebe_cfg_host.is_rolling != 0;
+}
+bool NS(TrackJob_get_default_elem_by_elem_config_rolling_flag)( const NS(TrackJobBase)* j )
+{
+    return j && j->ebe_rolling;
+}
+void NS(TrackJob_set_default_elem_by_elem_config_rolling_flag)( NS(TrackJobBase)* j, bool flag )
+{
+    if( j ) j->ebe_rolling = flag;
+}
+NS(elem_by_elem_order_t) NS(TrackJob_get_elem_by_elem_config_order)( const NS(TrackJobBase)* j )
+{
+    return ( j && j->plan.has_ebe ) ? j->ebe_cfg_host.order : ST_ELEM_BY_ELEM_ORDER_DEFAULT;
+}
+NS(elem_by_elem_order_t) NS(TrackJob_get_default_elem_by_elem_config_order)( const NS(TrackJobBase)* )
+{
+    return ST_ELEM_BY_ELEM_ORDER_DEFAULT;
+}
+void NS(TrackJob_set_default_elem_by_elem_config_order)( NS(TrackJobBase)*, NS(elem_by_elem_order_t) order )
+{
+    // one order exists (turn, element, particle), as in the reference
+    if( order != ST_ELEM_BY_ELEM_ORDER_DEFAULT ) g_err = "only the turn-element-particle order exists";
+}
+
+// -- stored buffers: capacity, const access; buffers by id ------------------
+NS(Buffer) const* NS(TrackJob_const_stored_buffer)( const NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    return ( j && id >= ST_ARCH_MIN_USER_DEFINED_BUFFER_ID ) ? j->buffer_by_id( id ) : nullptr;
+}
+NS(arch_size_t) NS(TrackJob_stored_buffers_capacity)( const NS(TrackJobBase)* j )
+{
+    return j ? j->stored.capacity() : 0u;
+}
+NS(arch_status_t) NS(TrackJob_reserve_stored_buffers_capacity)( NS(TrackJobBase)* j, NS(arch_size_t) cap )
+{
+    if( j == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    j->stored.reserve( cap );
+    return ST_ARCH_STATUS_SUCCESS;
+}
+NS(Buffer)* NS(TrackJob_buffer_by_buffer_id)( NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    return j ? j->buffer_by_id( id ) : nullptr;
+}
+NS(Buffer) const* NS(TrackJob_const_buffer_by_buffer_id)( const NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    return j ? j->buffer_by_id( id ) : nullptr;
+}
+bool NS(TrackJob_is_buffer_by_buffer_id)( const NS(TrackJobBase)* j, NS(arch_size_t) id )
+{
+    return j && j->buffer_by_id( id ) != nullptr;
+}
+bool NS(TrackJob_is_raw_memory_by_buffer_id)( const NS(TrackJobBase)*, NS(arch_size_t) ) { return false; }
+
+// -- assignment items: search, removal, pairs ------------------------------
+namespace
+{
+bool same_item( const NS(AssignAddressItem)& a, const NS(AssignAddressItem)& b )
+{
+    return std::memcmp( &a, &b, sizeof( a ) ) == 0;
+}
+
+NS(AssignAddressItem) make_item( NS(object_type_id_t) dt, NS(buffer_size_t) db, NS(buffer_size_t) di,
+    NS(buffer_size_t) dp, NS(object_type_id_t) st, NS(buffer_size_t) sb, NS(buffer_size_t) si,
+    NS(buffer_size_t) sp )
+{
+    NS(AssignAddressItem) it{};
+    it.dest_buffer_id = db; it.src_buffer_id = sb; it.dest_elem_type_id = dt; it.src_elem_type_id = st;
+    it.dest_elem_index = di; it.src_elem_index = si; it.dest_pointer_offset = dp; it.src_pointer_offset = sp;
+    return it;
+}
+
+// position of item k within its (dest, src) pair and the range of the pair
+void pair_range( const NS(TrackJobBase)* j, uint64_t dest, uint64_t src, size_t& b, size_t& e )
+{
+    b = e = 0;
+    bool found = false;
+    for( size_t k = 0; k < j->assign_items.size(); ++k )
+    {
+        bool const in = j->assign_items[ k ].dest_buffer_id == dest && j->assign_items[ k ].src_buffer_id == src;
+        if( in && !found ) { b = k; found = true; }
+        if( in ) e = k + 1;
+    }
+}
+}  // namespace
+
+NS(AssignAddressItem)* NS(TrackJob_add_assign_address_item_detail)( NS(TrackJobBase)* j,
+    NS(object_type_id_t) dt, NS(arch_size_t) db, NS(arch_size_t) di, NS(arch_size_t) dp,
+    NS(object_type_id_t) st, NS(arch_size_t) sb, NS(arch_size_t) si, NS(arch_size_t) sp )
+{
+    return NS(TrackJob_add_assign_address_item_detailed)( j, dt, db, di, dp, st, sb, si, sp );
+}
+
+NS(arch_status_t) NS(TrackJob_remove_assign_address_item)( NS(TrackJobBase)* j, NS(AssignAddressItem)* item )
+{
+    if( j == nullptr || item == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    for( size_t k = 0; k < j->assign_items.size(); ++k )
+        if( same_item( j->assign_items[ k ], *item ) )
+        {
+            j->assign_items.erase( j->assign_items.begin() + k );
+            ++j->items_version;
+            return ST_ARCH_STATUS_SUCCESS;
+        }
+    return ST_ARCH_STATUS_GENERAL_FAILURE;
+}
+
+NS(arch_status_t) NS(TrackJob_remove_assign_address_item_by_key_and_index)( NS(TrackJobBase)* j,
+    const NS(TrackJobDestSrcBufferIds)* key, NS(buffer_size_t) index )
+{
+    if( j == nullptr || key == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    size_t b, e;
+    pair_range( j, key->dest_buffer_id, key->src_buffer_id, b, e );
+    if( b + index >= e ) return ST_ARCH_STATUS_GENERAL_FAILURE;
+    j->assign_items.erase( j->assign_items.begin() + ( b + index ) );
+    ++j->items_version;
+    return ST_ARCH_STATUS_SUCCESS;
+}
+
+NS(AssignAddressItem) const* NS(TrackJob_ptr_assign_address_item)( const NS(TrackJobBase)* j,
+    const NS(AssignAddressItem)* item )
+{
+    if( j == nullptr || item == nullptr ) return nullptr;
+    for( const NS(AssignAddressItem)& it : j->assign_items ) if( same_item( it, *item ) ) return &it;
+    return nullptr;
+}
+
+bool NS(TrackJob_has_assign_address_item)( const NS(TrackJobBase)* j, const NS(AssignAddressItem)* item )
+{
+    return NS(TrackJob_ptr_assign_address_item)( j, item ) != nullptr;
+}
+
+NS(AssignAddressItem) const* NS(TrackJob_ptr_assign_address_item_detailed)( const NS(TrackJobBase)* j,
+    NS(object_type_id_t) dt, NS(buffer_size_t) db, NS(buffer_size_t) di, NS(buffer_size_t) dp,
+    NS(object_type_id_t) st, NS(buffer_size_t) sb, NS(buffer_size_t) si, NS(buffer_size_t) sp )
+{
+    NS(AssignAddressItem) const it = make_item( dt, db, di, dp, st, sb, si, sp );
+    return NS(TrackJob_ptr_assign_address_item)( j, &it );
+}
+
+bool NS(TrackJob_has_assign_address_item_detailed)( const NS(TrackJobBase)* j,
+    NS(object_type_id_t) dt, NS(buffer_size_t) db, NS(buffer_size_t) di, NS(buffer_size_t) dp,
+    NS(object_type_id_t) st, NS(buffer_size_t) sb, NS(buffer_size_t) si, NS(buffer_size_t) sp )
+{
+    return NS(TrackJob_ptr_assign_address_item_detailed)( j, dt, db, di, dp, st, sb, si, sp ) != nullptr;
+}
+
+NS(buffer_size_t) NS(TrackJob_index_of_assign_address_item)( const NS(TrackJobBase)* j,
+    const NS(AssignAddressItem)* item )
+{
+    if( j == nullptr || item == nullptr ) return 0u;
+    size_t b, e;
+    pair_range( j, item->dest_buffer_id, item->src_buffer_id, b, e );
+    for( size_t k = b; k < e; ++k ) if( same_item( j->assign_items[ k ], *item ) ) return k - b;
+    return e - b;
+}
+
+NS(buffer_size_t) NS(TrackJob_index_of_assign_address_item_detailed)( const NS(TrackJobBase)* j,
+    NS(object_type_id_t) dt, NS(buffer_size_t) db, NS(buffer_size_t) di, NS(buffer_size_t) dp,
+    NS(object_type_id_t) st, NS(buffer_size_t) sb, NS(buffer_size_t) si, NS(buffer_size_t) sp )
+{
+    NS(AssignAddressItem) const it = make_item( dt, db, di, dp, st, sb, si, sp );
+    return NS(TrackJob_index_of_assign_address_item)( j, &it );
+}
+
+NS(AssignAddressItem) const* NS(TrackJob_ptr_assign_address_item_by_index)( const NS(TrackJobBase)* j,
+    NS(buffer_size_t) dest, NS(buffer_size_t) src, NS(buffer_size_t) index )
+{
+    if( j == nullptr ) return nullptr;
+    size_t b, e;
+    pair_range( j, dest, src, b, e );
+    return ( b + index < e ) ? &j->assign_items[ b + index ] : nullptr;
+}
+
+bool NS(TrackJob_has_assign_item_by_index)( const NS(TrackJobBase)* j, NS(buffer_size_t) dest,
+    NS(buffer_size_t) src, NS(buffer_size_t) index )
+{
+    return NS(TrackJob_ptr_assign_address_item_by_index)( j, dest, src, index ) != nullptr;
+}
+
+static const std::vector< NS(AssignAddressItemDestSrcPair) >& item_pairs( const NS(TrackJobBase)* j )
+{
+    auto& pc = const_cast< NS(TrackJobBase)* >( j )->pair_cache;
+    pc.clear();
+    for( const NS(AssignAddressItem)& it : j->assign_items )
+    {
+        bool seen = false;
+        for( const auto& p : pc ) seen = seen || ( p.dest_buffer_id == it.dest_buffer_id &&
+                                                   p.src_buffer_id == it.src_buffer_id );
+        if( !seen ) pc.push_back( { it.dest_buffer_id, it.src_buffer_id } );
+    }
+    return pc;
+}
+
+NS(buffer_size_t) NS(TrackJob_num_distinct_available_assign_address_items_dest_src_pairs)(
+    const NS(TrackJobBase)* j )
+{
+    return j ? item_pairs( j ).size() : 0u;
+}
+NS(arch_size_t) NS(TrackJob_num_distinct_assign_items_dest_src_pairs)( const NS(TrackJobBase)* j )
+{
+    return NS(TrackJob_num_distinct_available_assign_address_items_dest_src_pairs)( j );
+}
+
+NS(buffer_size_t) NS(TrackJob_available_assign_address_items_dest_src_pairs)( const NS(TrackJobBase)* j,
+    NS(buffer_size_t) max_num, NS(TrackJobDestSrcBufferIds)* out )
+{
+    if( j == nullptr ) return 0u;
+    auto const& pc = item_pairs( j );
+    size_t const n = std::min< size_t >( pc.size(), max_num );
+    for( size_t k = 0; k < n && out; ++k ) out[ k ] = pc[ k ];
+    return n;
+}
+NS(arch_size_t) NS(TrackJob_assign_items_dest_src_pairs)( const NS(TrackJobBase)* j, NS(arch_size_t) max_num,
+    NS(AssignAddressItemDestSrcPair)* out )
+{
+    return NS(TrackJob_available_assign_address_items_dest_src_pairs)( j, max_num, out );
+}
+
+NS(TrackJobDestSrcBufferIds) const* NS(TrackJob_assign_item_dest_src_begin)( const NS(TrackJobBase)* j )
+{
+    return j ? item_pairs( j ).data() : nullptr;
+}
+NS(TrackJobDestSrcBufferIds) const* NS(TrackJob_assign_item_dest_src_end)( const NS(TrackJobBase)* j )
+{
+    return j ? j->pair_cache.data() + j->pair_cache.size() : nullptr;
+}
+
+NS(AssignAddressItem) const* NS(TrackJob_assign_items_dest_src_begin)( const NS(TrackJobBase)* j,
+    NS(arch_size_t) dest, NS(arch_size_t) src )
+{
+    if( j == nullptr ) return nullptr;
+    size_t b, e;
+    pair_range( j, dest, src, b, e );
+    return j->assign_items.data() + b;
+}
+NS(AssignAddressItem) const* NS(TrackJob_assign_items_dest_src_end)( const NS(TrackJobBase)* j,
+    NS(arch_size_t) dest, NS(arch_size_t) src )
+{
+    if( j == nullptr ) return nullptr;
+    size_t b, e;
+    pair_range( j, dest, src, b, e );
+    return j->assign_items.data() + e;
+}
+
+static NS(Object) const* pair_objects( const NS(TrackJobBase)* jc, uint64_t dest, uint64_t src, bool end )
+{
+    if( jc == nullptr ) return nullptr;
+    NS(TrackJobBase)* j = const_cast< NS(TrackJobBase)* >( jc );
+    // rebuilt when the pair or the items changed, so that begin and end of
+    // one pair come from the same buffer
+    if( j->item_objs == nullptr || j->item_objs_version != j->items_version ||
+        j->item_objs_dest != dest || j->item_objs_src != src )
+    {
+        if( j->item_objs ) NS(Buffer_delete)( j->item_objs );
+        j->item_objs = NS(Buffer_new)( 0 );
+        if( j->item_objs == nullptr ) return nullptr;
+        size_t b, e;
+        pair_range( j, dest, src, b, e );
+        for( size_t k = b; k < e; ++k )
+            if( NS(Buffer_add_object)( j->item_objs, &j->assign_items[ k ], sizeof( NS(AssignAddressItem) ),
+                                       65536u, 0, nullptr, nullptr, nullptr ) == nullptr ) return nullptr;
+        j->item_objs_version = j->items_version;
+        j->item_objs_dest = dest;
+        j->item_objs_src = src;
+    }
+    return end ? NS(Buffer_get_objects_end)( j->item_objs ) : NS(Buffer_get_objects_begin)( j->item_objs );
+}
+
+NS(Object) const* NS(TrackJob_assign_items_begin)( const NS(TrackJobBase)* j, NS(buffer_size_t) dest,
+    NS(buffer_size_t) src )
+{
+    return pair_objects( j, dest, src, false );
+}
+NS(Object) const* NS(TrackJob_assign_items_end)( const NS(TrackJobBase)* j, NS(buffer_size_t) dest,
+    NS(buffer_size_t) src )
+{
+    return pair_objects( j, dest, src, true );
+}
 
 }  // extern "C"

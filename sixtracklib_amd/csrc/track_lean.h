@@ -166,6 +166,93 @@ ST_DEV void wave_priority_leave( const WavePrio& wp, int64_t done )
     if( __lane_id() == 0 ) atomicAdd( wp.progress, ( unsigned long long )( wp.k - done ) );
 }
 
+// One turn of a lean until-turn lane through the record stream: the
+// element loop of track_lean_lane.  LANE_S: the lane carries its path
+// length s (added at every drift); without it every active lane started the
+// turn at s = +0 and s after a drift is the record's (lattice.h), written to
+// the lane only when it is lost.  The record at ``off``: its header and its
+// first two payload words are read one record ahead, together (for a drift:
+// its length and the path length after it).
+template< bool HEAVY, bool LANE_S, class PA >
+ST_DEV void lean_turn( const PA& pa, int64_t i, int64_t li, cptr_t lat, Lane& L, double chi,
+                       uint64_t& act, int32_t& status, int64_t start, int64_t at_turn )
+{
+    uint64_t s_rec = 0;                       // s after the last drift (bits)
+    uint32_t off = 0;
+    uint64_t h = rec_at( lat, 0 )[ 0 ];
+    uint64_t w2 = rec_at( lat, 0 )[ 2 ];
+    uint64_t w3 = rec_at( lat, 0 )[ 3 ];
+    for( ;; )
+    {
+        while( ( ( uint32_t )h & ST_OP_RARE_BIT ) == 0u )   // Drift / Multipole run
+        {
+            cptr_t const r = rec_at( lat, off );
+            uint32_t const noff = off + ST_REC_BYTES( h );
+            uint64_t const hn = rec_at( lat, noff )[ 0 ];
+            uint64_t const w2n = rec_at( lat, noff )[ 2 ];
+            uint64_t const w3n = rec_at( lat, noff )[ 3 ];
+            if( ( uint32_t )h & ST_OP_MP_BIT )
+            {
+                multipole( L, chi, r + ST_REC_PAYLOAD, ST_REC_AUX( h ), L.delta );
+            }
+            else
+            {
+                double const length = __longlong_as_double( ( long long )w2 );
+                drift_xz( L, length );
+                if constexpr( LANE_S ) L.s += length;
+                else s_rec = w3;
+                uint64_t const lost = limit_global_lost_mask( L ) & act;
+                if( lost != 0u )
+                {
+                    if( lane_in( lost ) )
+                    {
+                        if constexpr( !LANE_S ) L.s = __longlong_as_double( ( long long )s_rec );
+                        store_lane_state( pa.view(), i, L, 0, start + ri( r, 1 ), at_turn );
+                    }
+                    act &= ~lost;
+                }
+            }
+            off = noff;
+            h = hn;
+            w2 = w2n;
+            w3 = w3n;
+        }
+        unsigned const op = ST_REC_OP( h );
+        if( op == ST_OP_END ) break;
+        cptr_t const r = rec_at( lat, off );
+        uint32_t const noff = off + ST_REC_BYTES( h );
+        uint64_t const hn = rec_at( lat, noff )[ 0 ];
+        uint64_t const w2n = rec_at( lat, noff )[ 2 ];
+        uint64_t const w3n = rec_at( lat, noff )[ 3 ];
+        bool fail = false;
+        bool const ok = lean_other< HEAVY >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L, pa, li,
+                                             lane_in( act ), start + ri( r, 1 ), at_turn, fail );
+        if constexpr( !LANE_S )
+        {
+            if( op == ST_OP_DRIFT_EXACT )
+            {
+                L.s = 0.0;                    // drift_exact added the length
+                s_rec = w3;
+            }
+        }
+        uint64_t const lost = ballot( !ok ) & act;
+        if( lost != 0u )
+        {
+            if( lane_in( lost ) )
+            {
+                if constexpr( !LANE_S ) L.s = __longlong_as_double( ( long long )s_rec );
+                store_lane_state( pa.view(), i, L, 0, start + ri( r, 1 ), at_turn );
+                if( fail ) status = -1;
+            }
+            act &= ~lost;
+        }
+        off = noff;
+        h = hn;
+        w2 = w2n;
+        w3 = w3n;
+    }
+}
+
 // One lane of a lean until-turn launch (HEAVY: the lattice may also hold
 // the other maps, see lean_other).
 //   P          particle set; particle i = list[tid] with a list, else tid
@@ -209,55 +296,18 @@ ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
     wave_priority( wp, 0 );
     while( act != 0u )
     {
-        uint32_t off = 0;
-        uint64_t h = rec_at( lat, 0 )[ 0 ];
-        for( ;; )
-        {
-            while( ( ( uint32_t )h & ST_OP_RARE_BIT ) == 0u )   // Drift / Multipole run
-            {
-                cptr_t const r = rec_at( lat, off );
-                uint32_t const noff = off + ST_REC_BYTES( h );
-                uint64_t const hn = rec_at( lat, noff )[ 0 ];
-                if( ( uint32_t )h & ST_OP_MP_BIT )
-                {
-                    multipole( L, chi, r + ST_REC_PAYLOAD, ST_REC_AUX( h ), L.delta );
-                }
-                else
-                {
-                    drift( L, rd( r, ST_REC_PAYLOAD ) );
-                    uint64_t const lost = limit_global_lost_mask( L ) & act;
-                    if( lost != 0u )
-                    {
-                        if( lane_in( lost ) )
-                            store_lane_state( pa.view(), i, L, 0, start + ri( r, 1 ), at_turn );
-                        act &= ~lost;
-                    }
-                }
-                off = noff;
-                h = hn;
-            }
-            unsigned const op = ST_REC_OP( h );
-            if( op == ST_OP_END ) break;
-            cptr_t const r = rec_at( lat, off );
-            uint32_t const noff = off + ST_REC_BYTES( h );
-            uint64_t const hn = rec_at( lat, noff )[ 0 ];
-            bool fail = false;
-            bool const ok = lean_other< HEAVY >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L, pa, li,
-                                                 lane_in( act ), start + ri( r, 1 ), at_turn,
-                                                 fail );
-            uint64_t const lost = ballot( !ok ) & act;
-            if( lost != 0u )
-            {
-                if( lane_in( lost ) )
-                {
-                    store_lane_state( pa.view(), i, L, 0, start + ri( r, 1 ), at_turn );
-                    if( fail ) status = -1;
-                }
-                act &= ~lost;
-            }
-            off = noff;
-            h = hn;
-        }
+        // A turn on which every lane starts at s = +0 (every turn after a
+        // call's first, and the first when the particles start their turn
+        // there) needs no per-lane path length: after each drift it is the
+        // record's s (lattice.h), kept in a scalar register and written only
+        // when a lane is lost.  Lattices with monitors (HEAVY) store whole
+        // particles mid-turn and keep the per-lane s.
+        if constexpr( HEAVY )
+            lean_turn< true, true >( pa, i, li, lat, L, chi, act, status, start, at_turn );
+        else if( ballot( lane_in( act ) && __double_as_longlong( L.s ) != 0 ) != 0u )
+            lean_turn< false, true >( pa, i, li, lat, L, chi, act, status, start, at_turn );
+        else
+            lean_turn< false, false >( pa, i, li, lat, L, chi, act, status, start, at_turn );
         if( lane_in( act ) ) { ++at_turn; L.s = 0.0; }
         uint64_t const fin = ballot( at_turn >= turn_stop ) & act;
         if( fin != 0u )

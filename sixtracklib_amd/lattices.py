@@ -477,6 +477,15 @@ def kitchen_sink(seed=0, cbuffer=None, with_monitor=False):
     be.SCInterpolatedProfile(cbuffer=eb, number_of_particles=1e11, sigma_x=1e-3,
                              sigma_y=1.2e-3, length=1.0, line_density_prof_fallback=0.5)
     be.LimitRectEllipse(cbuffer=eb, max_x=0.02, max_y=0.02, a_squ=4e-4, b_squ=3e-4)
+    # orders above the unrolled 0-10 (the generic Horner loop), as in the
+    # LHC lattices of the reference's test data (orders up to 14); every
+    # order kicks by about 1e-7 at 1 cm, so that each coefficient shows in
+    # the result bits
+    scale = 1e-7 * np.array([math.factorial(k) / 0.01 ** k for k in range(15)])
+    be.Multipole(cbuffer=eb, knl=rng.normal(0.0, 1.0, 12) * scale[:12],
+                 ksl=rng.normal(0.0, 1.0, 12) * scale[:12])
+    be.Multipole(cbuffer=eb, knl=rng.normal(0.0, 1.0, 15) * scale,
+                 ksl=rng.normal(0.0, 1.0, 15) * scale)
     if with_monitor:
         be.BeamMonitor(cbuffer=eb, num_stores=4, start=0, skip=1, is_rolling=1)
     be.Drift(cbuffer=eb, length=0.3)

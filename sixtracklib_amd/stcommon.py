@@ -19,6 +19,19 @@ if not os.path.exists(SHARED_LIB):
                       "(or __graft_entry__.build())")
 
 
+def hip_runtime_path():
+    """Path of PyTorch's HIP runtime (libamdhip64.so next to libtorch_hip.so)
+    when torch is installed, else None."""
+    try:
+        spec = importlib.util.find_spec("torch")
+    except (ImportError, ValueError):
+        return None
+    if spec is None or not spec.submodule_search_locations:
+        return None
+    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
+    return path if os.path.exists(path) else None
+
+
 def _share_torch_hip_runtime():
     """PyTorch-ROCm wheels carry their own HIP runtime, which torch loads
     by file name (libamdhip64.so next to libtorch_hip.so).  Loaded first,
@@ -29,14 +42,8 @@ def _share_torch_hip_runtime():
     SIXTRACKLIB_AMD_SYSTEM_HIP=1 skips this."""
     if os.environ.get("SIXTRACKLIB_AMD_SYSTEM_HIP") == "1":
         return
-    try:
-        spec = importlib.util.find_spec("torch")
-    except (ImportError, ValueError):
-        return
-    if spec is None or not spec.submodule_search_locations:
-        return
-    path = os.path.join(list(spec.submodule_search_locations)[0], "lib", "libamdhip64.so")
-    if os.path.exists(path):
+    path = hip_runtime_path()
+    if path is not None:
         ct.CDLL(path, mode=ct.RTLD_GLOBAL)
 
 
@@ -158,6 +165,7 @@ _fn("st_TrackBatch_push", ct.c_int, [st_TrackBatch_p])
 _fn("st_TrackBatch_set_config", ct.c_int, [st_TrackBatch_p, ct.c_char_p])
 _fn("st_TrackBatch_get_config", ct.c_int64, [st_TrackBatch_p, ct.c_char_p, ct.c_int64])
 _fn("st_TrackBatch_autotune", ct.c_int, [st_TrackBatch_p, ct.c_int64])
+_fn("st_TrackBatch_autotune_window", ct.c_int, [st_TrackBatch_p, ct.c_int64, ct.c_int64])
 _fn("st_TrackBatch_num_seeds", ct.c_int64, [st_TrackBatch_p])
 _fn("st_TrackBatch_total_particles", ct.c_int64, [st_TrackBatch_p])
 _fn("st_TrackBatch_last_kernel_time_ms", ct.c_double, [st_TrackBatch_p])
@@ -213,6 +221,105 @@ _fn("st_TrackJob_assign_addresses", st_arch_status_t,
 _fn("st_Lattice_pack_with_tables", ct.c_int64,
     [st_uchar_p, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_int64, ct.c_void_p, ct.c_void_p,
      ct.c_void_p, ct.c_void_p])
+
+# -- the rest of the legacy job API (track_job.h:123-470) -----------------
+class st_ParticlesAddr(ct.Structure):
+    """particles/particles_addr.h:27-55"""
+    _fields_ = [("num_particles", ct.c_int64)] + [(n + "_addr", ct.c_uint64) for n in (
+        "q0", "mass0", "beta0", "gamma0", "p0c", "s", "x", "y", "px", "py", "zeta", "psigma",
+        "delta", "rpp", "rvv", "chi", "charge_ratio", "particle_id", "at_element_id", "at_turn",
+        "state")]
+
+
+class st_ElemByElemConfig(ct.Structure):
+    """output/elem_by_elem_config.h:39-53"""
+    _fields_ = [(n, ct.c_int64) for n in (
+        "order", "num_particles_to_store", "num_elements_to_store", "num_turns_to_store",
+        "min_particle_id", "min_element_id", "min_turn", "max_particle_id", "max_element_id",
+        "max_turn", "is_rolling")] + [("out_store_addr", ct.c_uint64)]
+
+
+class st_AssignAddressItemDestSrcPair(ct.Structure):
+    _fields_ = [("dest_buffer_id", ct.c_uint64), ("src_buffer_id", ct.c_uint64)]
+
+
+st_ParticlesAddr_p = ct.POINTER(st_ParticlesAddr)
+_fn("st_TrackJob_extract_device_id_str", ct.c_int, [ct.c_char_p, ct.c_char_p, st_buffer_size_t])
+_fn("st_TrackJob_sanitize_arch_str", ct.c_int, [ct.c_char_p, ct.c_char_p, st_buffer_size_t])
+_fn("st_TrackJob_sanitize_arch_str_inplace", ct.c_int, [ct.c_char_p, st_buffer_size_t])
+_fn("st_TrackJob_can_fetch_particles_addr", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_has_particles_addr", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_fetch_particle_addresses", st_arch_status_t, [st_TrackJob_p])
+_fn("st_TrackJob_clear_all_particle_addresses", st_arch_status_t, [st_TrackJob_p])
+_fn("st_TrackJob_particle_addresses", st_ParticlesAddr_p, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_get_elem_by_elem_config", ct.POINTER(st_ElemByElemConfig), [st_TrackJob_p])
+_fn("st_TrackJob_get_beam_monitor_indices_begin", st_buffer_size_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_num_particles_in_set", st_buffer_size_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_block_size", ct.c_int, [st_TrackJob_p])
+_fn("st_TrackJob_remove_assign_address_item", st_arch_status_t,
+    [st_TrackJob_p, st_AssignAddressItem_p])
+_fn("st_TrackJob_index_of_assign_address_item", st_buffer_size_t,
+    [st_TrackJob_p, st_AssignAddressItem_p])
+_fn("st_TrackJob_ptr_assign_address_item_by_index", st_AssignAddressItem_p,
+    [st_TrackJob_p, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJob_num_distinct_available_assign_address_items_dest_src_pairs", st_buffer_size_t,
+    [st_TrackJob_p])
+_fn("st_TrackJob_available_assign_address_items_dest_src_pairs", st_buffer_size_t,
+    [st_TrackJob_p, st_buffer_size_t, ct.POINTER(st_AssignAddressItemDestSrcPair)])
+_fn("st_TrackJob_assign_items_dest_src_begin", st_AssignAddressItem_p,
+    [st_TrackJob_p, st_arch_size_t, st_arch_size_t])
+_fn("st_TrackJob_assign_items_dest_src_end", st_AssignAddressItem_p,
+    [st_TrackJob_p, st_arch_size_t, st_arch_size_t])
+_fn("st_TrackJob_assign_items_begin", ct.c_void_p, [st_TrackJob_p, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJob_assign_items_end", ct.c_void_p, [st_TrackJob_p, st_buffer_size_t, st_buffer_size_t])
+
+# -- new-style job API and the CUDA job's constructors (track_job_new.h) ---
+st_TrackJobNew_p = ct.c_void_p
+st_ARCHITECTURE_HIP = 4
+_fn("st_TrackJobNew_create", st_TrackJobNew_p, [ct.c_char_p, ct.c_char_p])
+_fn("st_TrackJobNew_new", st_TrackJobNew_p, [ct.c_char_p, st_Buffer_p, st_Buffer_p, ct.c_char_p])
+_fn("st_TrackJobNew_new_with_output", st_TrackJobNew_p,
+    [ct.c_char_p, st_Buffer_p, st_Buffer_p, st_Buffer_p, st_buffer_size_t, ct.c_char_p])
+_fn("st_TrackJobNew_new_detailed", st_TrackJobNew_p,
+    [ct.c_char_p, st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p, st_Buffer_p,
+     st_buffer_size_t, ct.c_char_p])
+_fn("st_TrackJobNew_delete", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_track_until", st_track_status_t, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_track_elem_by_elem", st_track_status_t, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_track_line", st_track_status_t,
+    [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t, ct.c_bool])
+_fn("st_TrackJobNew_collect", ct.c_uint16, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_collect_particles", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_collect_output", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_push_particles", ct.c_uint16, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_arch_id", ct.c_uint64, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_arch_string", ct.c_char_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_uses_controller", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_uses_arguments", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_is_in_debug_mode", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_enable_debug_mode", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_output_buffer", st_Buffer_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_elem_by_elem_output_buffer_offset", st_buffer_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_total_num_of_particles", st_buffer_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_fetch_particle_addresses", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_particle_addresses", st_ParticlesAddr_p, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_add_assign_address_item_detailed", st_AssignAddressItem_p,
+    [st_TrackJobNew_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_total_num_assign_items", st_buffer_size_t, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_num_available_nodes", ct.c_uint64, [])
+_fn("st_CudaTrackJob_get_available_node_indices_list", ct.c_uint64,
+    [ct.c_uint64, ct.POINTER(ct.c_uint32)])
+_fn("st_CudaTrackJob_create", st_TrackJobNew_p, [])
+_fn("st_CudaTrackJob_new", st_TrackJobNew_p, [ct.c_char_p, st_Buffer_p, st_Buffer_p])
+_fn("st_CudaTrackJob_new_with_output", st_TrackJobNew_p,
+    [ct.c_char_p, st_Buffer_p, st_Buffer_p, st_Buffer_p, st_buffer_size_t])
+_fn("st_CudaTrackJob_new_detailed", st_TrackJobNew_p,
+    [ct.c_char_p, st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p, st_Buffer_p,
+     st_buffer_size_t, ct.c_char_p])
+_fn("st_CudaTrackJob_has_controller", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_particles_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_default_threads_per_block", ct.c_uint64, [st_TrackJobNew_p])
 
 _fn("st_Hip_num_devices", ct.c_int, [])
 _fn("st_Hip_last_error_str", ct.c_char_p, [])

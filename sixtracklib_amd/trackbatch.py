@@ -83,11 +83,17 @@ class TrackBatch:
         st.st_TrackBatch_get_config(self.ptr, buf, n + 1)
         return buf.value.decode()
 
-    def autotune(self, trial_turns=2):
-        """Time every candidate (block, waves_per_cu, queues) for
-        ``trial_turns`` turns on this batch (state restored afterwards) and
-        keep the fastest; returns the chosen configuration string."""
-        if st.st_TrackBatch_autotune(self.ptr, int(trial_turns)) != 0:
+    def autotune(self, trial_turns=2, skip_turns=None):
+        """Time every candidate (block, waves_per_cu, queues, particles per
+        lane, lattice staging) for ``trial_turns`` turns on this batch,
+        starting after ``skip_turns`` turns (default 10 x trial_turns, past
+        the early losses), keep the fastest and restore the batch state;
+        returns the chosen configuration string."""
+        if skip_turns is None:
+            rc = st.st_TrackBatch_autotune(self.ptr, int(trial_turns))
+        else:
+            rc = st.st_TrackBatch_autotune_window(self.ptr, int(skip_turns), int(trial_turns))
+        if rc != 0:
             raise RuntimeError("TrackBatch.autotune failed: %s" % st.last_error())
         return self.config
 

@@ -247,3 +247,85 @@ def test_trackbatch_argument_and_device_errors():
         pytest.skip("a device is present: construction would succeed")
     with pytest.raises(RuntimeError):
         TrackBatch([eb], [pb])
+
+
+def test_arch_and_device_id_strings():
+    """track_job.h:123-135: the device id of a configuration string and the
+    lower-case arch string (the reference's helpers)."""
+    import ctypes as ct
+    from sixtracklib_amd import stcommon as st
+    buf = ct.create_string_buffer(16)
+    assert st.st_TrackJob_extract_device_id_str(b"fma=1 device=3", buf, 16) == 0
+    assert buf.value == b"3"
+    assert st.st_TrackJob_extract_device_id_str(b"0.1;fma=0", buf, 16) == 0
+    assert buf.value == b"0.1"
+    assert st.st_TrackJob_extract_device_id_str(b"fma=1", buf, 16) == 0 and buf.value == b""
+    assert st.st_TrackJob_extract_device_id_str(b"device=12345", buf, 3) == -1
+    assert st.st_TrackJob_sanitize_arch_str(b" HiP ", buf, 16) == 0 and buf.value == b"hip"
+    inplace = ct.create_string_buffer(b"  CUDA", 16)
+    assert st.st_TrackJob_sanitize_arch_str_inplace(inplace, 16) == 0 and inplace.value == b"cuda"
+
+
+def test_assign_items_search_removal_and_pairs():
+    """Address-assignment bookkeeping (track_job.h:500-660) on a job made
+    without buffers (no device needed): items of one (dest, src) pair stay
+    next to each other in the order added; index, lookup by index, removal
+    and the distinct pairs."""
+    import ctypes as ct
+    from sixtracklib_amd import stcommon as st
+    job = st.st_TrackJob_create(b"hip", b"")
+    assert job
+    try:
+        add = st.st_TrackJob_add_assign_address_item_detailed
+        a = add(job, 257, 2, 5, 48, 258, 1024, 0, 0)
+        b = add(job, 11, 3, 0, 8, 1, 1, 0, 0)
+        c = add(job, 257, 2, 7, 48, 258, 1024, 1, 0)
+        assert a and b and c
+        assert st.st_TrackJob_total_num_assign_items(job) == 3
+        assert st.st_TrackJob_num_assign_items(job, 2, 1024) == 2
+        first = st.st_TrackJob_ptr_assign_address_item_by_index(job, 2, 1024, 0).contents
+        second = st.st_TrackJob_ptr_assign_address_item_by_index(job, 2, 1024, 1).contents
+        assert (first.dest_elem_index, second.dest_elem_index) == (5, 7)
+        assert not st.st_TrackJob_ptr_assign_address_item_by_index(job, 2, 1024, 2)
+        assert st.st_TrackJob_index_of_assign_address_item(job, ct.byref(second)) == 1
+        assert st.st_TrackJob_num_distinct_available_assign_address_items_dest_src_pairs(job) == 2
+        pairs = (st.st_AssignAddressItemDestSrcPair * 4)()
+        assert st.st_TrackJob_available_assign_address_items_dest_src_pairs(job, 4, pairs) == 2
+        assert [(p.dest_buffer_id, p.src_buffer_id) for p in pairs[:2]] == [(2, 1024), (3, 1)]
+        beg = st.st_TrackJob_assign_items_dest_src_begin(job, 2, 1024)
+        end = st.st_TrackJob_assign_items_dest_src_end(job, 2, 1024)
+        size = ct.sizeof(st.st_AssignAddressItem)
+        assert (ct.cast(end, ct.c_void_p).value - ct.cast(beg, ct.c_void_p).value) == 2 * size
+        ob, oe = st.st_TrackJob_assign_items_begin(job, 2, 1024), st.st_TrackJob_assign_items_end(job, 2, 1024)
+        assert (oe - ob) == 2 * 24                       # two NS(Object) descriptors
+        item = st.st_AssignAddressItem()
+        ct.memmove(ct.byref(item), ct.byref(first), size)
+        assert st.st_TrackJob_remove_assign_address_item(job, ct.byref(item)) == 0
+        assert st.st_TrackJob_num_assign_items(job, 2, 1024) == 1
+        assert st.st_TrackJob_remove_assign_address_item(job, ct.byref(item)) == -1
+    finally:
+        st.st_TrackJob_delete(job)
+
+
+def test_new_style_and_cuda_job_without_device():
+    """NS(TrackJobNew_*) / NS(CudaTrackJob_*) on jobs made without buffers:
+    "cuda" names the HIP job, the arch id is this library's, there are no
+    controller or argument objects; debug mode is remembered."""
+    from sixtracklib_amd import stcommon as st
+    for arch in (b"cuda", b"hip"):
+        job = st.st_TrackJobNew_create(arch, b"block=128")
+        assert job
+        assert st.st_TrackJobNew_get_arch_id(job) == st.st_ARCHITECTURE_HIP
+        assert st.st_TrackJobNew_get_arch_string(job) == b"hip"
+        assert not st.st_TrackJobNew_uses_controller(job) and not st.st_TrackJobNew_uses_arguments(job)
+        assert not st.st_TrackJobNew_is_in_debug_mode(job)
+        assert st.st_TrackJobNew_enable_debug_mode(job) == 0 and st.st_TrackJobNew_is_in_debug_mode(job)
+        assert st.st_TrackJobNew_add_assign_address_item_detailed(job, 257, 2, 0, 48, 258, 1024, 0, 0)
+        assert st.st_TrackJobNew_total_num_assign_items(job) == 1
+        st.st_TrackJobNew_delete(job)
+    assert not st.st_TrackJobNew_create(b"opencl", None)
+    cj = st.st_CudaTrackJob_create()
+    assert cj and not st.st_CudaTrackJob_has_controller(cj)
+    assert not st.st_CudaTrackJob_get_ptr_particles_arg(cj)
+    assert st.st_CudaTrackJob_default_threads_per_block(cj) == 256
+    st.st_TrackJobNew_delete(cj)

@@ -64,7 +64,9 @@ def _plain(rings, parts, turns, fma):
 @pytest.mark.parametrize("cfg", ["fma=0", "fma=0 queues=1 graph=0", "fma=0 turns_per_launch=2",
                                  "fma=0 block=64 waves_per_cu=16 turns_per_launch=3",
                                  "fma=0 ppl=2 waves_per_cu=20 turns_per_launch=2",
-                                 "fma=0 balance=1", "fma=0 balance=1 waves_per_cu=1 queues=8"])
+                                 "fma=0 balance=1", "fma=0 balance=1 waves_per_cu=1 queues=8",
+                                 "fma=0 staging=lds", "fma=0 staging=lds turns_per_launch=2 queues=1",
+                                 "fma=0 staging=lds balance=1 waves_per_cu=4"])
 def test_batch_matches_oracle_exact(cfg):
     from sixtracklib_amd.trackbatch import TrackBatch
     rings, parts = _workload()
@@ -86,16 +88,19 @@ def test_batch_matches_oracle_exact(cfg):
     assert 0 < lost < sum(COUNTS)
 
 
-def test_batch_fma_matches_plain_fma_bitwise():
+@pytest.mark.parametrize("cfg", ["fma=1 turns_per_launch=2", "fma=1 staging=lds turns_per_launch=2"])
+def test_batch_fma_matches_plain_fma_bitwise(cfg):
+    """FMA build: the batch (scalar-load or LDS-staged lattice) equals the
+    plain FMA job bit for bit."""
     from sixtracklib_amd.trackbatch import TrackBatch
     rings, parts = _workload()
     plain_in = [_copy(pb) for pb in parts]
     want = _plain(rings, plain_in, 5, fma=1)
-    batch = TrackBatch(rings, parts, config_str="fma=1 turns_per_launch=2")
+    batch = TrackBatch(rings, parts, config_str=cfg)
     batch.track_until(5)
     batch.collect()
     for s, pb in enumerate(parts):
-        assert_particles_equal(_pdict(pb), want[s], what="seed %d fma" % s)
+        assert_particles_equal(_pdict(pb), want[s], what="seed %d %s" % (s, cfg))
 
 
 def test_batch_autotune_keeps_state_and_results():
@@ -105,7 +110,7 @@ def test_batch_autotune_keeps_state_and_results():
     want = _plain(rings, plain_in, 4, fma=0)
     batch = TrackBatch(rings, parts, config_str="fma=0")
     cfg = batch.autotune(trial_turns=2)
-    assert "block=" in cfg and "queues=" in cfg
+    assert "block=" in cfg and "queues=" in cfg and "staging=" in cfg and "ppl=" in cfg
     batch.track_until(4)
     batch.collect()
     for s, pb in enumerate(parts):

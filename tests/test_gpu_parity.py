@@ -170,9 +170,11 @@ def test_fodo_ring_losses_in_steps_and_compaction():
     from sixtracklib_amd.trackjob import TrackJob
     got = {}
     pb4 = _copy(pb)
+    pb5 = _copy(pb)
     for key, buf, cfg in (("on", pb, "fma=0"), ("off", pb2, "fma=0 compact=0"),
                           ("chunked", pb3, "fma=0 turns_per_launch=2"),
-                          ("pairs", pb4, "fma=0 ppl=2 turns_per_launch=3")):
+                          ("pairs", pb4, "fma=0 ppl=2 turns_per_launch=3"),
+                          ("lds", pb5, "fma=0 staging=lds turns_per_launch=3")):
         job = TrackJob(eb, buf, config_str=cfg)
         for t in (3, 7, 10):
             assert job.track_until(t).last_track_status == 0
@@ -184,6 +186,7 @@ def test_fodo_ring_losses_in_steps_and_compaction():
     assert_particles_equal(got["off"], want, what="stepped, not gathered")
     assert_particles_equal(got["chunked"], want, what="stepped, 2 turns per launch")
     assert_particles_equal(got["pairs"], want, what="stepped, two particles per lane")
+    assert_particles_equal(got["lds"], want, what="stepped, lattice staged through LDS")
 
 
 @pytest.mark.parametrize("fma", [False, True])
