@@ -44,11 +44,11 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (KiB per dispatch; a 1 GiB device-to-device copy reads 524,308 KiB of
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
-# a full 25-turn track launch of the 1M-particle ring (8,192 tiles; mean of
-# the seven full launches of turns 0-100: 472,860 / 73,823 KiB) and one turn
-# of element-by-element tracking of the collective ring with 100k particles
-# (all windows of two 2-turn calls / 4: 151,306 / 205,193,746 KiB)
-TRACK_LAUNCH_HBM_BYTES = 472860 * 2048 + 73823 * 1024
+# a track launch of the plain 1000-turn job of the 1M-particle ring (mean of
+# its 63 launches: 484,421 / 61,047 KiB) and one turn of element-by-element
+# tracking of the collective ring with 100k particles (all windows of two
+# 2-turn calls / 4: 151,306 / 205,193,746 KiB)
+TRACK_LAUNCH_HBM_BYTES = 484421 * 2048 + 61047 * 1024
 EBE_TURN_HBM_BYTES = 151306 * 2048 + 205193746 * 1024
 
 
@@ -508,9 +508,9 @@ def main():
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "traffic": TRACK_LAUNCH_HBM_BYTES,
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of st_k_batch_track_fma "
-                                       "on this workload, mean of the seven full 25-turn launches "
-                                       "of turns 0-100 (8,192 tiles each), scaled by a 1 GiB device "
-                                       "copy (FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x)",
+                                       "on this workload, mean of the 63 launches of the 1000-turn "
+                                       "job, scaled by a 1 GiB device copy (FETCH_SIZE reads 0.5x, "
+                                       "WRITE_SIZE 1.0x)",
                      "algorithmic_hbm_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "batched": batched,

@@ -21,8 +21,11 @@
  * results bit-identical to the reference CPU tracker; 1: fused
  * multiply-add build: the energy fields and the path length rounded as
  * the reference rounds them, the transverse coordinates within about
- * 1e-13 of their rms after one turn of a 20,000-element ring, loss
- * elements and turns identical), "block=64..256",
+ * 1e-13 of their rms after one turn of a 20,000-element ring (more where
+ * the orbit swings far beyond the beam: 1.5e-11 on the reference's LHC
+ * lattice, whose 12 mm bumps dwarf a 0.2 mm beam), loss elements and
+ * turns identical on regular orbits, those whose exact-build loss does not
+ * move when the start moves by one ulp), "block=64..256",
  * "device=N", "ebe_rolling=0|1" (default 1, as the reference job),
  * "compact=0|1" (default 1: track_until first gathers the particles that
  * still have turns to go, so lost ones take no lanes; results identical),
@@ -45,7 +48,11 @@
  * ebe_rolling=1 it then keeps each particle's K most recent passes: record
  * (turn t, element e, particle p) goes to slot ((t - t0) * E + e) mod K * P
  * + p, the reference's rolling index with capacity K * P; with
- * ebe_rolling=0 it keeps the first K).
+ * ebe_rolling=0 it keeps the first K), "staging=scalar|lds" (default
+ * scalar: the until-turn kernel of lean lattices reads the lattice
+ * through scalar loads in every wave; lds: 256-lane workgroups copy it
+ * piece by piece into LDS and walk it together; one particle per lane;
+ * results identical).
  */
 #ifndef SIXTRACKLIB_AMD_TRACK_JOB_H__
 #define SIXTRACKLIB_AMD_TRACK_JOB_H__
