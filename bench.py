@@ -46,10 +46,10 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
 # a track launch of the plain 1000-turn job of the 1M-particle ring (mean of
 # its 63 launches: 484,421 / 61,047 KiB) and one turn of element-by-element
-# tracking of the collective ring with 100k particles (all windows of two
-# 2-turn calls / 4: 151,306 / 205,193,746 KiB)
+# tracking of the collective ring with 100k particles in the exact build
+# (one 10-turn call / 10: 6,505,271 / 2,051,917,580 KiB)
 TRACK_LAUNCH_HBM_BYTES = 484421 * 2048 + 61047 * 1024
-EBE_TURN_HBM_BYTES = 151306 * 2048 + 205193746 * 1024
+EBE_TURN_HBM_BYTES = (6505271 * 2048 + 2051917580 * 1024) // 10
 
 
 def sigma_at_start(eb):

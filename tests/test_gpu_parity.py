@@ -143,6 +143,24 @@ def test_lhc_like_ring_fma_one_turn(max_sigma):
     _assert_fma_close(got, want, "lhc_like_ring fma", nominal=False)
 
 
+def test_lhc_like_ring_exact_build_bitwise():
+    """The 20,000-element ring in the exact build: 1,024 particles of the
+    0-20 sigma polar grid for 40 turns, every field bit-identical to the
+    oracle; about 100 of them (the outer ones, beyond the dynamic
+    aperture) are lost, at 7 different turns."""
+    eb = lattices.lhc_like_ring(seed=0)
+    sig = lattices.twiss_sigma(beta=236.0)
+    pb, _ = lattices.polar_grid(16, 64, sigma_x=sig, sigma_y=sig, max_sigma=20.0, delta=2.7e-4)
+    ob = _copy(pb)
+    assert oracle_lib.track_until(ob, eb, 40, threads=16) == 0
+    got = _gpu_until(eb, pb, 40, fma=False)
+    want = _pdict(ob)
+    lost = want["state"] == 0
+    assert 50 < np.count_nonzero(lost) < 200
+    assert len(np.unique(want["at_turn"][lost])) >= 5
+    assert_particles_equal(got, want, what="lhc_like_ring exact")
+
+
 def test_fodo_ring_losses_match_oracle():
     """Large amplitudes: particles lost at the LimitRect and the global
     aperture; at_element / at_turn / state must match exactly."""
