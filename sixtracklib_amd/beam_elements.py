@@ -352,8 +352,11 @@ def bb6d_data(q_part, phi, alpha, delta_x, delta_y, N_part_per_slice,
     which is absent; this is a restatement of the published method
     (Hirata's boosted frame: boost the strong-beam sigma matrix by the
     crossing angle, order slices head first, boost each slice centroid).
-    Parity of this construction step with pysixtrack is unpinned; the
-    layout is pinned by the bbsimple/beambeam lattice fixtures.
+    Given the SixTrack inputs of the reference's three BeamBeam6D lenses
+    (tests/testdata/bbsimple and beambeam, fort.3 and beam_elements.bin),
+    it reproduces pysixtrack's words bit for bit; the closed orbit and the
+    subtracted kick are inputs here, as pysixtrack computes them from the
+    machine.
     """
     sphi, cphi, tphi = math.sin(phi), math.cos(phi), math.tan(phi)
     salpha, calpha = math.sin(alpha), math.cos(alpha)

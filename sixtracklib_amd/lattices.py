@@ -292,6 +292,25 @@ def adjoint_sublattice(seed=0, n_cells=20, tune=4.31, rel_err=1e-4, k2_scale=0.0
     return eb
 
 
+def gaussian_slice_centroids(n_slices, sigma_z):
+    """Longitudinal centroids of ``n_slices`` equal-charge slices of a
+    Gaussian bunch of rms length ``sigma_z``, tail first: slice k spans the
+    quantiles k / n .. (k + 1) / n, and its centroid is
+    sigma_z * n * (pdf(a) - pdf(b)) for its edges a < b in units of sigma
+    (the mean of a Gaussian restricted to [a, b]).  Two slices give
+    -+ sigma_z * sqrt(2 / pi), one gives 0."""
+    from statistics import NormalDist
+    nd = NormalDist()
+    edges = [nd.inv_cdf(k / n_slices) if 0 < k < n_slices else None
+             for k in range(n_slices + 1)]
+    cent = []
+    for k in range(n_slices):
+        pa = nd.pdf(edges[k]) if edges[k] is not None else 0.0
+        pb = nd.pdf(edges[k + 1]) if edges[k + 1] is not None else 0.0
+        cent.append(sigma_z * n_slices * (pa - pb))
+    return cent
+
+
 def collective_ring(seed=0, bunch_charge=1.15e11, sigma_ip=16.7e-6, beta_ip=0.55,
                     sigma_z=0.075, phi=142.5e-6, n_slices=5, n_sc=16, beta_arc=236.0,
                     sc_sigma_ratio=0.55, n_cells=400):
@@ -311,15 +330,7 @@ def collective_ring(seed=0, bunch_charge=1.15e11, sigma_ip=16.7e-6, beta_ip=0.55
       horizontally and ``sc_sigma_ratio`` times it vertically (an
       elliptical beam, so the kicks use the Faddeeva-function field).
     """
-    from statistics import NormalDist
-    nd = NormalDist()
-    edges = [nd.inv_cdf(k / n_slices) if 0 < k < n_slices else None
-             for k in range(n_slices + 1)]
-    cent = []
-    for k in range(n_slices):
-        pa = nd.pdf(edges[k]) if edges[k] is not None else 0.0
-        pb = nd.pdf(edges[k + 1]) if edges[k + 1] is not None else 0.0
-        cent.append(sigma_z * n_slices * (pa - pb))
+    cent = gaussian_slice_centroids(n_slices, sigma_z)
     sig_ip2 = sigma_ip * sigma_ip
     sig_p2 = (sigma_ip / beta_ip) ** 2
     sig_arc = twiss_sigma(beta=beta_arc)
