@@ -255,3 +255,44 @@ def test_fma_build_gradients_close_to_exact():
         out.append(X.grad.cpu().numpy())
     scale = np.max(np.abs(out[0]), axis=0)
     assert np.max(np.abs(out[1] - out[0]) / scale) < 1e-10
+
+
+S4 = np.array([[0.0, 1.0, 0.0, 0.0], [-1.0, 0.0, 0.0, 0.0],
+               [0.0, 0.0, 0.0, 1.0], [0.0, 0.0, -1.0, 0.0]])
+
+
+@pytest.mark.parametrize("turns", [1, 10])
+def test_adjoint_transverse_jacobian_is_symplectic(turns):
+    """With the cavity off (zero voltage: delta stays fixed) the sub-lattice
+    is drifts and thin multipoles, a symplectic map of (x, px, y, py).  The
+    adjoint's Jacobian (rows J^T e_c from gradient differences with the
+    target moved in the last turn, exact for a quadratic loss) has a
+    transverse block J4 with J4 S J4^T = S to rounding, after 1 and after
+    10 turns."""
+    eb = lattices.adjoint_sublattice(voltage=0.0)
+    _, p, coords = _beam(1, seed=33)
+    tr = _tracker(eb, turns, p, weights=(1.0,) * 6)
+    x0 = coords[0]
+
+    def grad_for(last):
+        tg = np.zeros((turns, 6))
+        tg[-1] = last
+        X = _dev(x0[None, :]).requires_grad_()
+        tr.loss(X, target=tg).backward()
+        return X.grad.cpu().numpy()[0]
+
+    g0 = grad_for(np.zeros(6))
+    J = np.zeros((6, 6))
+    for c in range(6):
+        e = np.zeros(6)
+        e[c] = 1e-3
+        J[c] = (g0 - grad_for(e)) / 2e-3
+    J4 = J[:4, :4]
+    assert np.max(np.abs(J4 - np.eye(4))) > 1e-1
+    big = float(np.max(np.abs(J4))) ** 2
+    resid = J4 @ S4 @ J4.T - S4
+    assert np.max(np.abs(resid)) < 1e-12 * big, (resid, J4)
+    # delta is a constant of the motion and zeta does not act back
+    np.testing.assert_array_equal(J[5, :5], np.zeros(5))
+    assert abs(J[5, 5] - 1.0) < 1e-10
+    assert np.max(np.abs(J[:4, 4])) == 0.0

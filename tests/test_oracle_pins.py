@@ -729,3 +729,50 @@ def test_stmath_and_libm_oracles_agree_on_c2_and_c3_samples():
     _two_builds(eb, pb, 2)
     pb, _ = lattices.matched_gaussian_beam(128, seed=5)
     _two_builds(lattices.collective_ring(), pb, 1)
+
+
+# ------------------------------------------------------------ symplecticity
+
+S4 = np.array([[0.0, 1.0, 0.0, 0.0], [-1.0, 0.0, 0.0, 0.0],
+               [0.0, 0.0, 0.0, 1.0], [0.0, 0.0, -1.0, 0.0]])
+
+
+def test_drift_multipole_map_is_symplectic():
+    """A line of drifts and thin multipoles (normal and skew, orders 0-4)
+    at fixed delta is a symplectic map of (x, px, y, py): its Jacobian J
+    satisfies J S J^T = S.  The drift moves x by px rpp L (rpp fixed), and
+    the multipole kick (-Re f, Im f) with f(x + iy) = sum (knl + i ksl)
+    z^n / n! is the gradient of Re F, F' = f, so its Jacobian is symmetric
+    (Cauchy-Riemann).  A transposed or mis-signed operand in the complex
+    Horner step breaks the symmetry; closed forms of single kicks may not
+    see a slip that is symmetric in x and y.  J from Richardson-extrapolated
+    central differences of the oracle, at three orbits with delta = 3e-4."""
+    rng = np.random.default_rng(41)
+    eb = _buf()
+    for k in range(12):
+        be.Drift(cbuffer=eb, length=float(rng.uniform(0.5, 3.0)))
+        order = k % 5
+        scale = np.array([1e-4, 5e-2, 1.0, 20.0, 300.0])[:order + 1]
+        be.Multipole(cbuffer=eb, knl=rng.normal(0, 1, order + 1) * scale,
+                     ksl=rng.normal(0, 1, order + 1) * scale)
+    fields = ("x", "px", "y", "py")
+    h = np.array([1e-6, 1e-7, 1e-6, 1e-7])
+    for orbit in ([1e-3, 2e-5, -5e-4, 1e-5], [-2e-3, -1e-5, 1.5e-3, 3e-5],
+                  [4e-4, 4e-5, 2e-4, -4e-5]):
+        x0 = np.array(orbit)
+        starts = []
+        for c in range(4):
+            for st in (1.0, -1.0, 0.5, -0.5):
+                v = x0.copy()
+                v[c] += st * h[c]
+                starts.append(v)
+        starts = np.array(starts)
+        got, _ = _run(eb, **{f: starts[:, k] for k, f in enumerate(fields)},
+                      delta=np.full(len(starts), 3e-4))
+        end = np.stack([got[f] for f in fields], 1).reshape(4, 4, 4)
+        d1 = (end[:, 0] - end[:, 1]) / (2 * h[:, None])
+        d2 = (end[:, 2] - end[:, 3]) / h[:, None]
+        J = ((4 * d2 - d1) / 3).T                       # [output][input]
+        assert np.max(np.abs(J - np.eye(4))) > 1e-2    # a real nonlinear map
+        resid = J @ S4 @ J.T - S4
+        assert np.max(np.abs(resid)) < 1e-7, (orbit, resid, J)
