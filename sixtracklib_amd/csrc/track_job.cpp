@@ -209,6 +209,7 @@ struct NS(TrackJobBase)
     int block = 256, device = 0;
     uint16_t collect_flags = ST_TRACK_JOB_COLLECT_DEFAULT_FLAGS;
     int32_t last_status = 0;
+    bool debug_mode = false;             // new-style API's debug mode (changes nothing)
 
     hipStream_t stream = nullptr;
     unsigned char* d_pbuf = nullptr; size_t pbuf_bytes = 0;
@@ -1047,6 +1048,17 @@ NS(Buffer)* NS(TrackJob_get_output_buffer)( NS(TrackJobBase)* j ) { return j ? j
 NS(track_status_t) NS(TrackJob_last_status)( const NS(TrackJobBase)* j ) { return j ? j->last_status : -1; }
 double NS(TrackJob_last_kernel_time_ms)( const NS(TrackJobBase)* j ) { return j ? ( double )j->last_kernel_ms : -1.0; }
 int NS(TrackJob_uses_fma)( const NS(TrackJobBase)* j ) { return ( j && j->fma ) ? 1 : 0; }
+// debug mode of the new-style API (track_job_new.cpp)
+extern "C" bool st_internal_job_debug_mode( const NS(TrackJobBase)* j )
+{
+    return j != nullptr && j->debug_mode;
+}
+
+extern "C" void st_internal_job_set_debug_mode( NS(TrackJobBase)* j, bool on )
+{
+    if( j != nullptr ) j->debug_mode = on;
+}
+
 int NS(TrackJob_uses_full_kernel)( const NS(TrackJobBase)* j ) { return ( j && j->lat_full ) ? 1 : 0; }
 int NS(TrackJob_block_size)( const NS(TrackJobBase)* j )
 {

@@ -12,6 +12,10 @@
 
 #include "sixtracklib_amd/track_job_new.h"
 
+// the debug-mode flag kept in the job (track_job.cpp)
+extern "C" bool st_internal_job_debug_mode( const NS(TrackJobBase)* j );
+extern "C" void st_internal_job_set_debug_mode( NS(TrackJobBase)* j, bool on );
+
 namespace
 {
 inline NS(TrackJobBase)* J( NS(TrackJobBaseNew)* j ) { return j; }
@@ -29,37 +33,6 @@ char const* hip_arch( char const* arch )
 NS(TrackJobBaseNew)* job_new_from( char const* arch, char const* config )
 {
     return NS(TrackJob_create)( hip_arch( arch ), config );
-}
-
-// debug mode flags per job (the mode changes nothing; it is remembered
-// for is_in_debug_mode)
-struct DebugFlags
-{
-    static constexpr int N = 256;
-    const void* job[ N ] = {};
-    bool on[ N ] = {};
-};
-DebugFlags g_debug;
-
-bool& debug_mode( const NS(TrackJobBaseNew)* j )
-{
-    static bool none = false;
-    int free_slot = -1;
-    for( int k = 0; k < DebugFlags::N; ++k )
-    {
-        if( g_debug.job[ k ] == j ) return g_debug.on[ k ];
-        if( free_slot < 0 && g_debug.job[ k ] == nullptr ) free_slot = k;
-    }
-    if( free_slot < 0 ) { none = false; return none; }
-    g_debug.job[ free_slot ] = j;
-    g_debug.on[ free_slot ] = false;
-    return g_debug.on[ free_slot ];
-}
-
-void forget_debug_mode( const NS(TrackJobBaseNew)* j )
-{
-    for( int k = 0; k < DebugFlags::N; ++k )
-        if( g_debug.job[ k ] == j ) { g_debug.job[ k ] = nullptr; g_debug.on[ k ] = false; }
 }
 
 // node id "P.D" or "D" -> "device=D" appended to the configuration
@@ -100,7 +73,6 @@ NS(TrackJobBaseNew)* NS(TrackJobNew_new_detailed)( char const* arch, NS(Buffer)*
 
 void NS(TrackJobNew_delete)( NS(TrackJobBaseNew)* job )
 {
-    forget_debug_mode( job );
     NS(TrackJob_delete)( J( job ) );
 }
 
@@ -290,20 +262,20 @@ NS(Buffer) const* NS(TrackJobNew_get_ptr_particle_addresses_buffer)( const NS(Tr
 
 bool NS(TrackJobNew_is_in_debug_mode)( const NS(TrackJobBaseNew)* job )
 {
-    return job != nullptr && debug_mode( job );
+    return st_internal_job_debug_mode( J( job ) );
 }
 
 NS(arch_status_t) NS(TrackJobNew_enable_debug_mode)( NS(TrackJobBaseNew)* job )
 {
     if( job == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
-    debug_mode( job ) = true;
+    st_internal_job_set_debug_mode( J( job ), true );
     return ST_ARCH_STATUS_SUCCESS;
 }
 
 NS(arch_status_t) NS(TrackJobNew_disable_debug_mode)( NS(TrackJobBaseNew)* job )
 {
     if( job == nullptr ) return ST_ARCH_STATUS_GENERAL_FAILURE;
-    debug_mode( job ) = false;
+    st_internal_job_set_debug_mode( J( job ), false );
     return ST_ARCH_STATUS_SUCCESS;
 }
 

@@ -321,6 +321,292 @@ _fn("st_CudaTrackJob_has_controller", ct.c_bool, [st_TrackJobNew_p])
 _fn("st_CudaTrackJob_get_ptr_particles_arg", ct.c_void_p, [st_TrackJobNew_p])
 _fn("st_CudaTrackJob_default_threads_per_block", ct.c_uint64, [st_TrackJobNew_p])
 
+# -- the rest of the C API of the headers above -----------------------------
+# (buffer.h, particles.h, beam_elements.h, track_job.h, track_job_new.h; the
+# adjoint's entry points are bound in adjoint.py with its problem struct)
+
+
+class st_NodeId(ct.Structure):
+    """NS(NodeId) of track_job_new.h: a HIP device as a node"""
+    _fields_ = [("platform_id", ct.c_int64), ("device_id", ct.c_int64),
+                ("node_index", ct.c_uint32)]
+
+
+_fn("st_Buffer_clear", ct.c_int, [st_Buffer_p, ct.c_bool])
+_fn("st_Buffer_reserve", ct.c_int,
+    [st_Buffer_p, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_Buffer_get_slot_size", st_buffer_size_t, [st_Buffer_p])
+_fn("st_Buffer_get_num_of_slots", st_buffer_size_t, [st_Buffer_p])
+_fn("st_Buffer_get_num_of_dataptrs", st_buffer_size_t, [st_Buffer_p])
+_fn("st_Buffer_get_data_begin", st_uchar_p, [st_Buffer_p])
+_fn("st_Buffer_get_const_data_begin", st_uchar_p, [st_Buffer_p])
+_fn("st_Buffer_owns_datastore", ct.c_bool, [st_Buffer_p])
+_fn("st_Buffer_get_objects_begin", ct.c_void_p, [st_Buffer_p])
+_fn("st_Buffer_get_objects_end", ct.c_void_p, [st_Buffer_p])
+_fn("st_Buffer_get_const_object", ct.c_void_p, [st_Buffer_p, st_buffer_size_t])
+_fn("st_Buffer_add_object", ct.c_void_p,
+    [st_Buffer_p, ct.c_void_p, st_buffer_size_t, st_object_type_id_t, st_buffer_size_t, st_buffer_size_p,
+     st_buffer_size_p, st_buffer_size_p])
+_fn("st_Object_get_type_id", st_object_type_id_t, [ct.c_void_p])
+_fn("st_Object_get_begin_addr", st_buffer_addr_t, [ct.c_void_p])
+_fn("st_Object_get_size", st_buffer_size_t, [ct.c_void_p])
+_fn("st_Particles_add_copy", ct.c_void_p, [st_Buffer_p, ct.c_void_p])
+_fn("st_Particles_buffer_get_const_particles", ct.c_void_p, [st_Buffer_p, st_buffer_size_t])
+_fn("st_Particles_get_num_of_particles", ct.c_int64, [ct.c_void_p])
+_fn("st_Particles_copy", ct.c_int, [ct.c_void_p, ct.c_void_p])
+_fn("st_Particles_find_min_max_attributes", ct.c_int,
+    [ct.c_void_p, ct.POINTER(st_particle_index_t), ct.POINTER(st_particle_index_t),
+     ct.POINTER(st_particle_index_t), ct.POINTER(st_particle_index_t), ct.POINTER(st_particle_index_t),
+     ct.POINTER(st_particle_index_t)])
+_fn("st_Multipole_new", ct.c_void_p, [st_Buffer_p, ct.c_int64])
+_fn("st_RFMultipole_add", ct.c_void_p,
+    [st_Buffer_p, ct.c_int64, ct.c_double, ct.c_double, ct.c_double, st_buffer_addr_t, st_buffer_addr_t])
+_fn("st_XYShift_add", ct.c_void_p, [st_Buffer_p, ct.c_double, ct.c_double])
+_fn("st_SRotation_add", ct.c_void_p, [st_Buffer_p, ct.c_double, ct.c_double])
+_fn("st_LimitEllipse_add", ct.c_void_p, [st_Buffer_p, ct.c_double, ct.c_double])
+_fn("st_LimitRectEllipse_add", ct.c_void_p, [st_Buffer_p, ct.c_double, ct.c_double, ct.c_double, ct.c_double])
+_fn("st_DipoleEdge_add", ct.c_void_p, [st_Buffer_p, ct.c_double, ct.c_double])
+_fn("st_BeamMonitor_add", ct.c_void_p,
+    [st_Buffer_p, ct.c_int64, ct.c_int64, ct.c_int64, st_buffer_addr_t, ct.c_int64, ct.c_int64, ct.c_bool,
+     ct.c_bool])
+_fn("st_BeamBeam4D_add", ct.c_void_p, [st_Buffer_p, ct.POINTER(ct.c_double), ct.c_uint64])
+_fn("st_BeamBeam6D_add", ct.c_void_p, [st_Buffer_p, ct.POINTER(ct.c_double), ct.c_uint64])
+_fn("st_SCCoasting_add", ct.c_void_p,
+    [st_Buffer_p, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double,
+     ct.c_double, ct.c_bool])
+_fn("st_SCQGaussProfile_add", ct.c_void_p,
+    [st_Buffer_p, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double, ct.c_double,
+     ct.c_double, ct.c_double, ct.c_bool])
+_fn("st_TrackJob_enable_collect_beam_elements", None, [st_TrackJob_p])
+_fn("st_TrackJob_disable_collect_beam_elements", None, [st_TrackJob_p])
+_fn("st_TrackJob_is_collecting_beam_elements", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_get_collect_flags", ct.c_uint16, [st_TrackJob_p])
+_fn("st_TrackJob_set_collect_flags", None, [st_TrackJob_p, ct.c_uint16])
+_fn("st_TrackJob_collect_particles_addresses", None, [st_TrackJob_p])
+_fn("st_TrackJob_push_particles_addresses", None, [st_TrackJob_p])
+_fn("st_TrackJob_fetch_particles_addr", st_arch_status_t, [st_TrackJob_p])
+_fn("st_TrackJob_clear_particle_addresses", st_arch_status_t, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_clear_particles_addr", st_arch_status_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_clear_all_particles_addr", st_arch_status_t, [st_TrackJob_p])
+_fn("st_TrackJob_particles_addr", st_ParticlesAddr_p, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_get_particles_addr_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_const_particles_addr_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_particles_addr_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_reset", ct.c_bool, [st_TrackJob_p, st_Buffer_p, st_Buffer_p, st_Buffer_p])
+_fn("st_TrackJob_reset_with_output", ct.c_bool,
+    [st_TrackJob_p, st_Buffer_p, st_Buffer_p, st_Buffer_p, st_buffer_size_t])
+_fn("st_TrackJob_reset_detailed", ct.c_bool,
+    [st_TrackJob_p, st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p, st_Buffer_p,
+     st_buffer_size_t])
+_fn("st_TrackJob_reset_particle_set", ct.c_bool,
+    [st_TrackJob_p, st_Buffer_p, st_buffer_size_t, st_Buffer_p, st_Buffer_p])
+_fn("st_TrackJob_select_particle_set", ct.c_bool, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_assign_output_buffer", ct.c_bool, [st_TrackJob_p, st_Buffer_p])
+_fn("st_TrackJob_get_type_id", ct.c_int64, [st_TrackJob_p])
+_fn("st_TrackJob_get_particle_set_indices_begin", st_buffer_size_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_particle_set_indices_end", st_buffer_size_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_particle_set_index", st_buffer_size_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_get_num_particles_in_sets_begin", st_buffer_size_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_num_particles_in_sets_end", st_buffer_size_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_particles_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_const_particles_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_beam_elements_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_const_beam_elements_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_const_output_buffer", st_Buffer_p, [st_TrackJob_p])
+_fn("st_TrackJob_has_beam_monitors", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_get_beam_monitor_indices_end", st_buffer_size_p, [st_TrackJob_p])
+_fn("st_TrackJob_get_beam_monitor_index", st_buffer_size_t, [st_TrackJob_p, st_buffer_size_t])
+_fn("st_TrackJob_has_elem_by_elem_config", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_is_elem_by_elem_config_rolling", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_get_default_elem_by_elem_config_rolling_flag", ct.c_bool, [st_TrackJob_p])
+_fn("st_TrackJob_set_default_elem_by_elem_config_rolling_flag", None, [st_TrackJob_p, ct.c_bool])
+_fn("st_TrackJob_get_elem_by_elem_config_order", ct.c_int64, [st_TrackJob_p])
+_fn("st_TrackJob_get_default_elem_by_elem_config_order", ct.c_int64, [st_TrackJob_p])
+_fn("st_TrackJob_set_default_elem_by_elem_config_order", None, [st_TrackJob_p, ct.c_int64])
+_fn("st_TrackJob_const_stored_buffer", st_Buffer_p, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_stored_buffers_capacity", st_arch_size_t, [st_TrackJob_p])
+_fn("st_TrackJob_reserve_stored_buffers_capacity", st_arch_status_t, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_buffer_by_buffer_id", st_Buffer_p, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_const_buffer_by_buffer_id", st_Buffer_p, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_is_buffer_by_buffer_id", ct.c_bool, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_is_raw_memory_by_buffer_id", ct.c_bool, [st_TrackJob_p, st_arch_size_t])
+_fn("st_TrackJob_add_assign_address_item_detail", st_AssignAddressItem_p,
+    [st_TrackJob_p, st_object_type_id_t, st_arch_size_t, st_arch_size_t, st_arch_size_t, st_object_type_id_t,
+     st_arch_size_t, st_arch_size_t, st_arch_size_t])
+_fn("st_TrackJob_remove_assign_address_item_by_key_and_index", st_arch_status_t,
+    [st_TrackJob_p, ct.POINTER(st_AssignAddressItemDestSrcPair), st_buffer_size_t])
+_fn("st_TrackJob_has_assign_address_item", ct.c_bool, [st_TrackJob_p, st_AssignAddressItem_p])
+_fn("st_TrackJob_has_assign_item_by_index", ct.c_bool,
+    [st_TrackJob_p, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJob_has_assign_address_item_detailed", ct.c_bool,
+    [st_TrackJob_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJob_index_of_assign_address_item_detailed", st_buffer_size_t,
+    [st_TrackJob_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJob_ptr_assign_address_item", st_AssignAddressItem_p, [st_TrackJob_p, st_AssignAddressItem_p])
+_fn("st_TrackJob_ptr_assign_address_item_detailed", st_AssignAddressItem_p,
+    [st_TrackJob_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJob_num_distinct_assign_items_dest_src_pairs", st_arch_size_t, [st_TrackJob_p])
+_fn("st_TrackJob_assign_items_dest_src_pairs", st_arch_size_t,
+    [st_TrackJob_p, st_arch_size_t, ct.POINTER(st_AssignAddressItemDestSrcPair)])
+_fn("st_TrackJob_assign_item_dest_src_begin", ct.POINTER(st_AssignAddressItemDestSrcPair), [st_TrackJob_p])
+_fn("st_TrackJob_assign_item_dest_src_end", ct.POINTER(st_AssignAddressItemDestSrcPair), [st_TrackJob_p])
+_fn("st_TrackJobNew_collect_detailed", ct.c_uint16, [st_TrackJobNew_p, ct.c_uint16])
+_fn("st_TrackJobNew_collect_beam_elements", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_collect_debug_flag", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_collect_particles_addresses", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_enable_collect_particles", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_disable_collect_particles", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_is_collecting_particles", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_enable_collect_beam_elements", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_disable_collect_beam_elements", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_is_collecting_beam_elements", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_enable_collect_output", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_disable_collect_output", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_is_collecting_output", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_collect_flags", ct.c_uint16, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_set_collect_flags", None, [st_TrackJobNew_p, ct.c_uint16])
+_fn("st_TrackJobNew_requires_collecting", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_push", ct.c_uint16, [st_TrackJobNew_p, ct.c_uint16])
+_fn("st_TrackJobNew_push_beam_elements", ct.c_uint16, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_push_output", ct.c_uint16, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_can_fetch_particle_addresses", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_has_particle_addresses", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_clear_particle_addresses", st_arch_status_t, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_clear_all_particle_addresses", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_ptr_particle_addresses_buffer", st_Buffer_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_disable_debug_mode", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_clear", None, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_reset", st_arch_status_t, [st_TrackJobNew_p, st_Buffer_p, st_Buffer_p, st_Buffer_p])
+_fn("st_TrackJobNew_reset_particle_set", st_arch_status_t,
+    [st_TrackJobNew_p, st_Buffer_p, st_buffer_size_t, st_Buffer_p, st_Buffer_p])
+_fn("st_TrackJobNew_reset_with_output", st_arch_status_t,
+    [st_TrackJobNew_p, st_Buffer_p, st_Buffer_p, st_Buffer_p, st_buffer_size_t])
+_fn("st_TrackJobNew_reset_detailed", st_arch_status_t,
+    [st_TrackJobNew_p, st_Buffer_p, st_buffer_size_t, st_buffer_size_p, st_Buffer_p, st_Buffer_p,
+     st_buffer_size_t])
+_fn("st_TrackJobNew_select_particle_set", st_arch_status_t, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_assign_output_buffer", st_arch_status_t, [st_TrackJobNew_p, st_Buffer_p])
+_fn("st_TrackJobNew_has_arch_string", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_has_config_str", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_config_str", ct.c_char_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_num_particle_sets", st_buffer_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_particle_set_indices_begin", st_buffer_size_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_particle_set_indices_end", st_buffer_size_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_particle_set_index", st_buffer_size_t, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_get_min_particle_id", st_particle_index_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_max_particle_id", st_particle_index_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_min_element_id", st_particle_index_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_max_element_id", st_particle_index_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_min_initial_turn_id", st_particle_index_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_max_initial_turn_id", st_particle_index_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_particles_buffer", st_Buffer_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_const_particles_buffer", st_Buffer_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_beam_elements_buffer", st_Buffer_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_const_beam_elements_buffer", st_Buffer_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_has_output_buffer", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_owns_output_buffer", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_has_elem_by_elem_output", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_has_beam_monitor_output", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_beam_monitor_output_buffer_offset", st_buffer_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_num_elem_by_elem_turns", st_buffer_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_const_output_buffer", st_Buffer_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_has_beam_monitors", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_num_beam_monitors", st_buffer_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_beam_monitor_indices_begin", st_buffer_size_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_beam_monitor_indices_end", st_buffer_size_p, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_beam_monitor_index", st_buffer_size_t, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_has_elem_by_elem_config", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_elem_by_elem_config", ct.POINTER(st_ElemByElemConfig), [st_TrackJobNew_p])
+_fn("st_TrackJobNew_is_elem_by_elem_config_rolling", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_default_elem_by_elem_config_rolling_flag", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_set_default_elem_by_elem_config_rolling_flag", None, [st_TrackJobNew_p, ct.c_bool])
+_fn("st_TrackJobNew_get_elem_by_elem_config_order", ct.c_int64, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_get_default_elem_by_elem_config_order", ct.c_int64, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_set_default_elem_by_elem_config_order", None, [st_TrackJobNew_p, ct.c_int64])
+_fn("st_TrackJobNew_add_assign_address_item", st_AssignAddressItem_p,
+    [st_TrackJobNew_p, st_AssignAddressItem_p])
+_fn("st_TrackJobNew_remove_assign_address_item", st_arch_status_t, [st_TrackJobNew_p, st_AssignAddressItem_p])
+_fn("st_TrackJobNew_remove_assign_address_item_by_key_and_index", st_arch_status_t,
+    [st_TrackJobNew_p, ct.POINTER(st_AssignAddressItemDestSrcPair), st_buffer_size_t])
+_fn("st_TrackJobNew_has_assign_address_item", ct.c_bool, [st_TrackJobNew_p, st_AssignAddressItem_p])
+_fn("st_TrackJobNew_has_assign_item_by_index", ct.c_bool,
+    [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_has_assign_address_item_detailed", ct.c_bool,
+    [st_TrackJobNew_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_index_of_assign_address_item_detailed", st_buffer_size_t,
+    [st_TrackJobNew_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_index_of_assign_address_item", st_buffer_size_t,
+    [st_TrackJobNew_p, st_AssignAddressItem_p])
+_fn("st_TrackJobNew_has_assign_items", ct.c_bool, [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_num_assign_items", st_buffer_size_t,
+    [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_ptr_assign_address_item", st_AssignAddressItem_p,
+    [st_TrackJobNew_p, st_AssignAddressItem_p])
+_fn("st_TrackJobNew_ptr_assign_address_item_detailed", st_AssignAddressItem_p,
+    [st_TrackJobNew_p, st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t,
+     st_object_type_id_t, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_ptr_assign_address_item_by_index", st_AssignAddressItem_p,
+    [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_num_distinct_available_assign_address_items_dest_src_pairs", st_buffer_size_t,
+    [st_TrackJobNew_p])
+_fn("st_TrackJobNew_available_assign_address_items_dest_src_pairs", st_buffer_size_t,
+    [st_TrackJobNew_p, st_buffer_size_t, ct.POINTER(st_AssignAddressItemDestSrcPair)])
+_fn("st_TrackJobNew_buffer_by_buffer_id", st_Buffer_p, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_const_buffer_by_buffer_id", st_Buffer_p, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_is_buffer_by_buffer_id", ct.c_bool, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_is_raw_memory_by_buffer_id", ct.c_bool, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_assign_items_begin", ct.c_void_p, [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_assign_items_end", ct.c_void_p, [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_assign_item_dest_src_begin", ct.POINTER(st_AssignAddressItemDestSrcPair),
+    [st_TrackJobNew_p])
+_fn("st_TrackJobNew_assign_item_dest_src_end", ct.POINTER(st_AssignAddressItemDestSrcPair),
+    [st_TrackJobNew_p])
+_fn("st_TrackJobNew_commit_address_assignments", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_assign_all_addresses", st_arch_status_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_assign_addresses", st_arch_status_t,
+    [st_TrackJobNew_p, st_buffer_size_t, st_buffer_size_t])
+_fn("st_TrackJobNew_stored_buffers_capacity", st_arch_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_reserve_stored_buffers_capacity", st_arch_status_t, [st_TrackJobNew_p, st_arch_size_t])
+_fn("st_TrackJobNew_has_stored_buffers", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_num_stored_buffers", st_arch_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_min_stored_buffer_id", st_arch_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_max_stored_buffer_id", st_arch_size_t, [st_TrackJobNew_p])
+_fn("st_TrackJobNew_create_stored_buffer", st_arch_size_t, [st_TrackJobNew_p, st_buffer_size_t])
+_fn("st_TrackJobNew_add_stored_buffer", st_arch_size_t, [st_TrackJobNew_p, st_Buffer_p, ct.c_bool, ct.c_bool])
+_fn("st_TrackJobNew_owns_stored_buffer", ct.c_bool, [st_TrackJobNew_p, st_arch_size_t])
+_fn("st_TrackJobNew_remove_stored_buffer", st_arch_status_t, [st_TrackJobNew_p, st_arch_size_t])
+_fn("st_TrackJobNew_stored_buffer", st_Buffer_p, [st_TrackJobNew_p, st_arch_size_t])
+_fn("st_TrackJobNew_const_stored_buffer", st_Buffer_p, [st_TrackJobNew_p, st_arch_size_t])
+_fn("st_TrackJobNew_push_stored_buffer", st_arch_status_t, [st_TrackJobNew_p, st_arch_size_t])
+_fn("st_TrackJobNew_collect_stored_buffer", st_arch_status_t, [st_TrackJobNew_p, st_arch_size_t])
+_fn("st_CudaTrackJob_get_available_node_ids_list", ct.c_uint64, [ct.c_uint64, ct.POINTER(st_NodeId)])
+_fn("st_CudaTrackJob_new_from_config_str", st_TrackJobNew_p, [ct.c_char_p])
+_fn("st_CudaTrackJob_get_ptr_controller", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_const_controller", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_default_track_threads_per_block", st_arch_size_t, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_has_particles_arg", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_const_particles_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_has_beam_elements_arg", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_beam_elements_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_const_beam_elements_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_has_output_arg", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_output_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_const_output_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_has_elem_by_elem_config_arg", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_elem_by_elem_config_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_const_elem_by_elem_config_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_has_debug_register_arg", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_debug_register_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_const_debug_register_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_has_particles_addr_arg", ct.c_bool, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_particles_addr_arg", ct.c_void_p, [st_TrackJobNew_p])
+_fn("st_CudaTrackJob_get_ptr_const_particles_addr_arg", ct.c_void_p, [st_TrackJobNew_p])
+
 _fn("st_Hip_num_devices", ct.c_int, [])
 _fn("st_Hip_last_error_str", ct.c_char_p, [])
 

@@ -37,6 +37,34 @@ def test_library_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_python_bindings_cover_every_declared_function():
+    """stcommon binds every function the headers declare (the adjoint's in
+    adjoint.py), each with as many argument types as the C parameters."""
+    from sixtracklib_amd import stcommon as st
+    from sixtracklib_amd import adjoint  # noqa: F401  (binds the st_Adjoint_* entry points)
+    missing, wrong = [], []
+    for fn in sorted(os.listdir(INCLUDE)):
+        if not fn.endswith(".h"):
+            continue
+        txt = re.sub(r"/\*.*?\*/", "", open(os.path.join(INCLUDE, fn)).read(), flags=re.S)
+        txt = re.sub(r"#define ST_CUDA_TRACK_JOB_ARG_GETTERS.*?#undef ST_CUDA_TRACK_JOB_ARG_GETTERS",
+                     "", txt, flags=re.S)
+        for m in re.finditer(r"ST_API[^;{]*?NS\((\w+)\)\s*\((.*?)\)\s*;", txt, re.S):
+            name, args = "st_" + m.group(1), " ".join(m.group(2).split())
+            n_args = 0 if args in ("", "void") else args.count(",") + 1
+            f = getattr(st, name, None) if fn != "adjoint.h" else getattr(st.sixtracklib, name)
+            if f is None:
+                missing.append(name)
+            elif f.argtypes is None or len(f.argtypes) != n_args:
+                wrong.append((name, n_args, None if f.argtypes is None else len(f.argtypes)))
+    assert not missing, missing
+    assert not wrong, wrong
+    for w in ("particles", "beam_elements", "output", "elem_by_elem_config", "debug_register",
+              "particles_addr"):
+        for form in ("has_%s_arg", "get_ptr_%s_arg", "get_ptr_const_%s_arg"):
+            assert hasattr(st, "st_CudaTrackJob_" + form % w)
+
+
 def test_c_buffer_roundtrip_with_python_buffer(tmp_path):
     from sixtracklib_amd import stcommon as st
     b = st.st_Buffer_new(0)
@@ -324,6 +352,18 @@ def test_new_style_and_cuda_job_without_device():
         assert st.st_TrackJobNew_total_num_assign_items(job) == 1
         st.st_TrackJobNew_delete(job)
     assert not st.st_TrackJobNew_create(b"opencl", None)
+    # the debug mode belongs to its job
+    jobs = [st.st_TrackJobNew_create(b"hip", None) for _ in range(300)]
+    assert all(jobs)
+    for k, j in enumerate(jobs):
+        if k % 3 == 0:
+            assert st.st_TrackJobNew_enable_debug_mode(j) == 0
+    assert [bool(st.st_TrackJobNew_is_in_debug_mode(j)) for j in jobs] == \
+        [k % 3 == 0 for k in range(len(jobs))]
+    assert st.st_TrackJobNew_disable_debug_mode(jobs[0]) == 0
+    assert not st.st_TrackJobNew_is_in_debug_mode(jobs[0])
+    for j in jobs:
+        st.st_TrackJobNew_delete(j)
     cj = st.st_CudaTrackJob_create()
     assert cj and not st.st_CudaTrackJob_has_controller(cj)
     assert not st.st_CudaTrackJob_get_ptr_particles_arg(cj)
