@@ -67,29 +67,9 @@ class TrackJob:
         idx = (st.st_buffer_size_t * len(particle_set_indices))(*particle_set_indices)
         self._set_indices = idx
 
-        # size and build the output buffer on the Python side, like the
-        # reference binding (trackjob.py:1355-1405)
-        self._output_buffer = None
-        self._c_obuf = None
-        sizes = (ct.c_int64 * 4096)()
-        has_ebe = ct.c_int64(0)
-        n = st.st_TrackJob_required_output_sets(
-            self._c_pbuf, len(particle_set_indices), idx, self._c_ebuf,
-            self._until_turn_ebe, sizes, 4096, ct.byref(has_ebe),
-            st.string_to_encoded_ctypes_str(cfg))
-        if n < 0:
-            raise RuntimeError("cannot determine the output requirements: " + st.last_error())
-        if output_buffer is not None:
-            self._output_buffer = get_cbuffer_from_obj(output_buffer)
-        elif n > 0:
-            total = sum(int(sizes[k]) for k in range(n))
-            ob = CBuffer(max_slots=22 * n + 21 * total + 64, max_objects=n + 4,
-                         max_pointers=21 * n + 4)
-            for k in range(n):
-                Particles(cbuffer=ob, num_particles=int(sizes[k]))
-            self._output_buffer = ob
-        if self._output_buffer is not None:
-            self._c_obuf = self._map(self._output_buffer)
+        self._config_str = cfg
+        self._output_buffer, self._c_obuf = self._prepare_output(
+            self._c_pbuf, idx, self._c_ebuf, self._until_turn_ebe, output_buffer)
 
         self.ptr_st_track_job = st.st_TrackJob_new_detailed(
             arch.encode(), self._c_pbuf, len(particle_set_indices), idx,
@@ -100,6 +80,28 @@ class TrackJob:
             raise RuntimeError("unable to create TrackJob: " + st.last_error())
         self._last_status = 0
         self._last_track_status = 0
+
+    def _prepare_output(self, c_pbuf, idx, c_ebuf, until_turn_ebe, output_buffer):
+        """The output buffer (CBuffer, mapped NS(Buffer)*): the caller's, or
+        one sized and built on the Python side like the reference binding
+        (trackjob.py:1355-1405), or none when nothing is stored."""
+        sizes = (ct.c_int64 * 4096)()
+        has_ebe = ct.c_int64(0)
+        n = st.st_TrackJob_required_output_sets(
+            c_pbuf, len(idx), idx, c_ebuf, until_turn_ebe, sizes, 4096, ct.byref(has_ebe),
+            st.string_to_encoded_ctypes_str(self._config_str))
+        if n < 0:
+            raise RuntimeError("cannot determine the output requirements: " + st.last_error())
+        ob = None
+        if output_buffer is not None:
+            ob = get_cbuffer_from_obj(output_buffer)
+        elif n > 0:
+            total = sum(int(sizes[k]) for k in range(n))
+            ob = CBuffer(max_slots=22 * n + 21 * total + 64, max_objects=n + 4,
+                         max_pointers=21 * n + 4)
+            for k in range(n):
+                Particles(cbuffer=ob, num_particles=int(sizes[k]))
+        return ob, (self._map(ob) if ob is not None else None)
 
     def _map(self, cbuf):
         p = st.st_Buffer_new_on_memory(cbuf.pointer, cbuf.words.nbytes)
@@ -389,3 +391,206 @@ class TrackJob:
             raise RuntimeError("Unable to perform assignment of address items: " +
                                st.last_error())
         return self
+
+    @staticmethod
+    def _item_struct(item):
+        if isinstance(item, st.st_AssignAddressItem):
+            return item
+        return st.st_AssignAddressItem(*[int(getattr(item, f)) for f, _ in
+                                         st.st_AssignAddressItem._fields_])
+
+    @staticmethod
+    def _detail(*args):
+        return None not in args
+
+    def has_assign_item(self, item=None, dest_buffer_id=None, src_buffer_id=None, index=None,
+                        dest_elem_type_id=None, dest_elem_index=None, dest_pointer_offset=None,
+                        src_elem_type_id=None, src_elem_index=None, src_pointer_offset=None):
+        """python/sixtracklib/trackjob.py: an item given whole, by its eight
+        fields, or by (dest_buffer_id, src_buffer_id, index)."""
+        j = self.ptr_st_track_job
+        if item is not None:
+            return bool(st.st_TrackJob_has_assign_address_item(j, ct.byref(self._item_struct(item))))
+        if self._detail(dest_elem_type_id, dest_buffer_id, dest_elem_index, dest_pointer_offset,
+                        src_elem_type_id, src_buffer_id, src_elem_index, src_pointer_offset):
+            return bool(st.st_TrackJob_has_assign_address_item_detailed(
+                j, int(dest_elem_type_id), int(dest_buffer_id), int(dest_elem_index),
+                int(dest_pointer_offset), int(src_elem_type_id), int(src_buffer_id),
+                int(src_elem_index), int(src_pointer_offset)))
+        if self._detail(dest_buffer_id, src_buffer_id, index):
+            return bool(st.st_TrackJob_has_assign_item_by_index(
+                j, int(dest_buffer_id), int(src_buffer_id), int(index)))
+        return False
+
+    def index_of_assign_address_item(self, item=None, dest_buffer_id=None, src_buffer_id=None,
+                                     dest_elem_type_id=None, dest_elem_index=None,
+                                     dest_pointer_offset=None, src_elem_type_id=None,
+                                     src_elem_index=None, src_pointer_offset=None):
+        """Index of the item among those of its (dest, src) pair; None when
+        the job has no such item."""
+        j = self.ptr_st_track_job
+        if item is not None:
+            it = self._item_struct(item)
+            if not st.st_TrackJob_has_assign_address_item(j, ct.byref(it)):
+                return None
+            return int(st.st_TrackJob_index_of_assign_address_item(j, ct.byref(it)))
+        if self._detail(dest_elem_type_id, dest_buffer_id, dest_elem_index, dest_pointer_offset,
+                        src_elem_type_id, src_buffer_id, src_elem_index, src_pointer_offset):
+            args = (int(dest_elem_type_id), int(dest_buffer_id), int(dest_elem_index),
+                    int(dest_pointer_offset), int(src_elem_type_id), int(src_buffer_id),
+                    int(src_elem_index), int(src_pointer_offset))
+            if not st.st_TrackJob_has_assign_address_item_detailed(j, *args):
+                return None
+            return int(st.st_TrackJob_index_of_assign_address_item_detailed(j, *args))
+        return None
+
+    def ptr_assign_address_item(self, item=None, dest_elem_type_id=None, dest_buffer_id=None,
+                                dest_elem_index=None, dest_pointer_offset=None,
+                                src_elem_type_id=None, src_buffer_id=None, src_elem_index=None,
+                                src_pointer_offset=None, index=None):
+        """Pointer to the job's copy of an item (st_NullAssignAddressItem
+        when there is none)."""
+        j = self.ptr_st_track_job
+        if item is not None:
+            return st.st_TrackJob_ptr_assign_address_item(j, ct.byref(self._item_struct(item)))
+        if self._detail(dest_elem_type_id, dest_buffer_id, dest_elem_index, dest_pointer_offset,
+                        src_elem_type_id, src_buffer_id, src_elem_index, src_pointer_offset):
+            return st.st_TrackJob_ptr_assign_address_item_detailed(
+                j, int(dest_elem_type_id), int(dest_buffer_id), int(dest_elem_index),
+                int(dest_pointer_offset), int(src_elem_type_id), int(src_buffer_id),
+                int(src_elem_index), int(src_pointer_offset))
+        if self._detail(dest_buffer_id, src_buffer_id, index):
+            return st.st_TrackJob_ptr_assign_address_item_by_index(
+                j, int(dest_buffer_id), int(src_buffer_id), int(index))
+        return st.st_NullAssignAddressItem
+
+    # -- particle addresses (python/sixtracklib/trackjob.py:1575-1620) ------
+    @property
+    def can_fetch_particle_addresses(self):
+        return bool(st.st_TrackJob_can_fetch_particles_addr(self.ptr_st_track_job))
+
+    @property
+    def has_particle_addresses(self):
+        return bool(st.st_TrackJob_has_particles_addr(self.ptr_st_track_job))
+
+    def fetch_particle_addresses(self):
+        """Device addresses of the fields of every particle set of the job's
+        copy (valid until the job is reset or deleted)."""
+        self._last_status = st.st_TrackJob_fetch_particle_addresses(self.ptr_st_track_job)
+        if self._last_status != 0:
+            raise RuntimeError("unsuccessful fetching particle addresses op; status:%d"
+                               % self._last_status)
+        return self
+
+    def get_particle_addresses(self, particle_set_index=0):
+        return st.st_TrackJob_particle_addresses(self.ptr_st_track_job, int(particle_set_index))
+
+    def clear_particle_addresses(self, particle_set_index=0):
+        self._last_status = st.st_TrackJob_clear_particle_addresses(self.ptr_st_track_job,
+                                                                    int(particle_set_index))
+        if self._last_status != 0:
+            raise RuntimeError("unsuccessful clearing of particle addresses op; "
+                               "particle_set_index=%d, status:%d"
+                               % (particle_set_index, self._last_status))
+        return self
+
+    def clear_all_particle_addresses(self):
+        self._last_status = st.st_TrackJob_clear_all_particle_addresses(self.ptr_st_track_job)
+        if self._last_status != 0:
+            raise RuntimeError("unsuccessful clearing all particle addresses op; status:%d"
+                               % self._last_status)
+        return self
+
+    # -- reset, identity, nodes --------------------------------------------
+    def reset(self, beam_elements_buffer, particles_buffer, until_turn_elem_by_elem=0,
+              output_buffer=None):
+        """python/sixtracklib/trackjob.py: the job takes new particle and
+        lattice buffers (and an output buffer, made here when one is needed
+        and none is given), keeping its configuration."""
+        old = self._c_buffers
+        self._c_buffers = []
+        try:
+            eb = get_cbuffer_from_obj(beam_elements_buffer)
+            pb = get_cbuffer_from_obj(particles_buffer)
+            c_pbuf, c_ebuf = self._map(pb), self._map(eb)
+            ob, c_obuf = self._prepare_output(c_pbuf, self._set_indices, c_ebuf,
+                                              int(until_turn_elem_by_elem), output_buffer)
+            if not st.st_TrackJob_reset_with_output(self.ptr_st_track_job, c_pbuf, c_ebuf, c_obuf,
+                                                    int(until_turn_elem_by_elem)):
+                raise RuntimeError("unable to reset the TrackJob: " + st.last_error())
+        except Exception:
+            self._free_buffers()
+            self._c_buffers = old
+            raise
+        for p in old:
+            st.st_Buffer_delete(p)
+        self._beam_elements_buffer, self._particles_buffer = eb, pb
+        self._c_pbuf, self._c_ebuf = c_pbuf, c_ebuf
+        self._output_buffer, self._c_obuf = ob, c_obuf
+        self._until_turn_ebe = int(until_turn_elem_by_elem)
+        return self
+
+    def type(self):
+        return int(st.st_TrackJob_get_type_id(self.ptr_st_track_job))
+
+    @property
+    def arch_id(self):
+        return int(st.st_TrackJob_get_type_id(self.ptr_st_track_job))
+
+    @staticmethod
+    def enabled_archs():
+        return ["hip"]
+
+    @staticmethod
+    def num_all_nodes(arch_str=None):
+        """HIP devices for "hip" (or "cuda", the reference's GPU job); 0 for
+        other architectures, as the reference gives for disabled ones."""
+        if arch_str is not None and str(arch_str).strip().lower() in _GPU_ARCHS + ("cuda",):
+            return int(st.st_Hip_num_devices())
+        return 0
+
+    @staticmethod
+    def num_available_nodes(arch_str=None, env_var_name=None, filter_str=None):
+        return TrackJob.num_all_nodes(arch_str)
+
+    @staticmethod
+    def print_nodes(arch_str, all=False, filter_str=None, env_var_name=None):
+        n = TrackJob.num_all_nodes(arch_str)
+        print("%d %s node%s" % (n, str(arch_str).strip().lower(), "" if n == 1 else "s"))
+        for k in range(n):
+            print("  node 0.%d: HIP device %d" % (k, k))
+
+    # OpenCL programs, kernels, controllers and arguments do not exist for
+    # this architecture; the reference answers the same way for every
+    # architecture but OpenCL
+    @property
+    def allows_add_program(self):
+        return False
+
+    def add_program(self, path_to_program_file, compile_options):
+        raise RuntimeError("Can not add a program to this TrackJob")
+
+    @property
+    def allows_enable_kernel(self):
+        return False
+
+    def enable_kernel(self, program_id, kernel_name):
+        raise RuntimeError("Can not enable a kernel at this TrackJob")
+
+    @property
+    def controller(self):
+        raise RuntimeError("TrackJob has no controller for this architecture")
+
+    def argument_by_buffer_id(self, buffer_id):
+        raise RuntimeError("unable to get argument for buffer on arch %s" % self._arch)
+
+    def stored_buffer_argument(self, buffer_id):
+        raise RuntimeError("unable to get argument for stored buffer on arch %s" % self._arch)
+
+    @property
+    def opencl_context_addr(self):
+        return 0
+
+    @property
+    def opencl_queue_addr(self):
+        return 0

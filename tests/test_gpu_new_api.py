@@ -163,3 +163,61 @@ def test_particle_addresses_point_at_the_device_copy():
         st.st_TrackJobNew_delete(job)
         st.st_Buffer_delete(pbuf)
         st.st_Buffer_delete(ebuf)
+
+
+def test_python_trackjob_reference_methods():
+    """The Python TrackJob's remaining methods of the reference binding
+    (python/sixtracklib/trackjob.py): particle addresses, reset with new
+    buffers, assign-item queries, identity and node queries, and the
+    OpenCL-only features refused as the reference refuses them."""
+    from sixtracklib_amd import stcommon as st
+    from sixtracklib_amd.trackjob import TrackJob
+    eb = lattices.fodo_ring()
+    pb = _beam()
+    job = TrackJob(eb, pb, config_str="fma=0")
+    assert job.type() == job.arch_id
+    assert TrackJob.enabled_archs() == ["hip"]
+    assert TrackJob.num_all_nodes("hip") >= 1 and TrackJob.num_all_nodes("opencl") == 0
+    assert TrackJob.num_available_nodes("cuda") == TrackJob.num_all_nodes("hip")
+    assert not job.allows_add_program and not job.allows_enable_kernel
+    with pytest.raises(RuntimeError):
+        job.add_program("k.cl", "")
+    with pytest.raises(RuntimeError):
+        job.controller
+    assert job.opencl_context_addr == 0
+    # particle addresses: the x field of the job's device copy holds x
+    assert job.can_fetch_particle_addresses
+    job.fetch_particle_addresses()
+    assert job.has_particle_addresses
+    pa = job.get_particle_addresses(0).contents
+    assert pa.num_particles == 777 and pa.x_addr != 0
+    hip = ct.CDLL(st.hip_runtime_path() or "libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ct.c_void_p, ct.c_void_p, ct.c_size_t, ct.c_int]
+    hip.hipMemcpy.restype = ct.c_int
+    host = np.empty(777, dtype=np.float64)
+    assert hip.hipMemcpy(host.ctypes.data, pa.x_addr, host.nbytes, 2) == 0      # D2H
+    np.testing.assert_array_equal(host, _pdict(pb)["x"])
+    job.clear_all_particle_addresses()
+    assert not job.get_particle_addresses(0)
+    # reset onto other particles and another lattice, then track: the
+    # oracle's result for the new buffers
+    eb2 = lattices.lhc_like_ring(seed=2, n_cells=8, tune=8.31 * 8 / 40)
+    pb2 = lattices.gaussian_beam(300, seed=4, sigma_x=2e-4, sigma_y=2e-4, sigma_px=2e-6,
+                                 sigma_py=2e-6)[0]
+    ob = _copy(pb2)
+    assert oracle_lib.track_until(ob, eb2, 3) == 0
+    job.reset(eb2, pb2)
+    assert job.particles_buffer is pb2 and job.beam_elements_buffer is eb2
+    job.track_until(3).collect()
+    assert_particles_equal(_pdict(pb2), _pdict(ob), what="after reset")
+    # assign-item queries
+    item = st.st_AssignAddressItem(dest_elem_type_id=257, dest_buffer_id=2, dest_elem_index=0,
+                                   dest_pointer_offset=48, src_elem_type_id=258,
+                                   src_buffer_id=1024, src_elem_index=0, src_pointer_offset=0)
+    assert not job.has_assign_item(item) and job.index_of_assign_address_item(item) is None
+    job.add_assign_address_item(item)
+    assert job.has_assign_item(item)
+    assert job.has_assign_item(dest_buffer_id=2, src_buffer_id=1024, index=0)
+    assert job.index_of_assign_address_item(item) == 0
+    assert job.ptr_assign_address_item(dest_buffer_id=2, src_buffer_id=1024, index=0)
+    del job
