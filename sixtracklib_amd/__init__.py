@@ -6,8 +6,8 @@ libsixtrack_amd.so on first use and raise if it is missing.
 """
 from .cobjects import CBuffer
 from .cobject import CObject, CField, get_object
-from .particles import (Particles, ParticlesSet, makeCopy, calcParticlesDifference,
-                        compareParticlesDifference)
+from .particles import (Particles, ParticlesSet, ParticlesAddr, makeCopy,
+                        calcParticlesDifference, compareParticlesDifference)
 from .beam_elements import (Drift, DriftExact, Multipole, RFMultipole, Cavity, XYShift,
                             SRotation, BeamMonitor, LimitRect, LimitEllipse,
                             LimitRectEllipse, DipoleEdge, BeamBeam4D, BeamBeam6D,
@@ -15,19 +15,32 @@ from .beam_elements import (Drift, DriftExact, Multipole, RFMultipole, Cavity, X
                             LineDensityProfileData, Elements,
                             append_beam_monitors_to_lattice)
 
-__all__ = ["CBuffer", "CObject", "CField", "get_object", "Particles", "ParticlesSet",
+__all__ = ["CBuffer", "CObject", "CField", "get_object", "Particles", "ParticlesSet", "ParticlesAddr",
            "makeCopy", "calcParticlesDifference", "compareParticlesDifference",
            "Drift", "DriftExact", "Multipole", "RFMultipole", "Cavity", "XYShift",
            "SRotation", "BeamMonitor", "LimitRect", "LimitEllipse", "LimitRectEllipse",
            "DipoleEdge", "BeamBeam4D", "BeamBeam6D", "SCCoasting", "SCQGaussProfile",
            "SCInterpolatedProfile", "LineDensityProfileData", "Elements",
-           "append_beam_monitors_to_lattice", "TrackJob", "TrackBatch"]
+           "append_beam_monitors_to_lattice", "TrackJob", "TrackJobBaseNew", "CudaTrackJob",
+           "TrackBatch", "Buffer", "AssignAddressItem", "supports"]
 
 
 def __getattr__(name):
     if name == "TrackJob":
         from .trackjob import TrackJob
         return TrackJob
+    if name == "TrackJobBaseNew":
+        from .trackjob import TrackJobBaseNew
+        return TrackJobBaseNew
+    if name == "CudaTrackJob":
+        from .cuda import CudaTrackJob
+        return CudaTrackJob
+    if name in ("Buffer", "AssignAddressItem"):
+        from . import buffer
+        return getattr(buffer, name)
+    if name == "supports":
+        from .config_helper import supports
+        return supports
     if name == "TrackBatch":
         from .trackbatch import TrackBatch
         return TrackBatch

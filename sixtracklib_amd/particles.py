@@ -172,38 +172,74 @@ def compareParticlesDifference(lhs, rhs, abs_treshold=None):
 
 class ParticlesSet:
     """A CBuffer holding several Particles objects
-    (python/sixtracklib/particles.py ``ParticlesSet``)."""
+    (python/sixtracklib/particles.py ``ParticlesSet``): ``particles`` lists
+    the objects of the buffer, so a set made on an existing buffer (a job's
+    ``output``, a file) sees every Particles object in it."""
 
     element_types = {"Particles": Particles}
 
     def __init__(self, cbuffer=None):
         self.cbuffer = cbuffer if cbuffer is not None else CBuffer()
-        self.particles = []
+
+    @property
+    def particles(self):
+        cb = self.cbuffer
+        return [Particles._from_buffer(cb, i) for i in range(cb.n_objects)
+                if cb.get_object_typeid(i) == Particles._typeid]
 
     def Particles(self, **kwargs):
-        p = Particles(cbuffer=self.cbuffer, **kwargs)
-        self.particles.append(p)
-        return p
+        return Particles(cbuffer=self.cbuffer, **kwargs)
 
     @classmethod
     def fromfile(cls, filename):
-        cb = CBuffer.fromfile(filename)
-        ps = cls(cbuffer=cb)
-        for i in range(cb.n_objects):
-            if cb.get_object_typeid(i) == Particles._typeid:
-                ps.particles.append(get_object(cb, i))
-        return ps
+        return cls(cbuffer=CBuffer.fromfile(filename))
 
     def tofile(self, filename):
         self.cbuffer.tofile(filename)
+
+    to_file = tofile
 
     @property
     def num_particles(self):
         return sum(p.num_particles for p in self.particles)
 
 
+class ParticlesAddr(CObject):
+    """NS(ParticlesAddr) (particles/particles_addr.h:27-55; Python
+    python/sixtracklib/particles.py ``ParticlesAddr``): the particle count
+    and the device address of each of the 21 field arrays of one particle
+    set, object type 512.  A job fills such records with
+    ``fetch_particle_addresses``."""
+    _typeid = 512
+    num_particles = CField(0, "int64", default=0)
+    q0 = CField(1, "uint64", default=0)
+    mass0 = CField(2, "uint64", default=0)
+    beta0 = CField(3, "uint64", default=0)
+    gamma0 = CField(4, "uint64", default=0)
+    p0c = CField(5, "uint64", default=0)
+    s = CField(6, "uint64", default=0)
+    x = CField(7, "uint64", default=0)
+    y = CField(8, "uint64", default=0)
+    px = CField(9, "uint64", default=0)
+    py = CField(10, "uint64", default=0)
+    zeta = CField(11, "uint64", default=0)
+    psigma = CField(12, "uint64", default=0)
+    delta = CField(13, "uint64", default=0)
+    rpp = CField(14, "uint64", default=0)
+    rvv = CField(15, "uint64", default=0)
+    chi = CField(16, "uint64", default=0)
+    charge_ratio = CField(17, "uint64", default=0)
+    particle_id = CField(18, "uint64", default=0)
+    at_element = CField(19, "uint64", default=0)
+    at_turn = CField(20, "uint64", default=0)
+    state = CField(21, "uint64", default=0)
+
+
 def particles_in(cbuffer, index=0):
-    """The ``index``-th object of ``cbuffer`` as :class:`Particles`."""
+    """The ``index``-th object of ``cbuffer`` (a CBuffer, or a ParticlesSet
+    such as a job's ``output``) as :class:`Particles`."""
+    if isinstance(cbuffer, ParticlesSet):
+        cbuffer = cbuffer.cbuffer
     if cbuffer.get_object_typeid(index) != Particles._typeid:
         raise TypeError("object %d is not a Particles object" % index)
     return Particles._from_buffer(cbuffer, index)

@@ -19,7 +19,7 @@ import numpy as np
 
 from . import stcommon as st
 from .cobjects import CBuffer
-from .particles import Particles
+from .particles import Particles, ParticlesSet
 
 _GPU_ARCHS = ("hip", "amd", "amdgpu", "gpu")
 
@@ -145,7 +145,10 @@ class TrackJob:
 
     @property
     def output(self):
-        return self._output_buffer
+        """python/sixtracklib/trackjob.py: the output buffer as a
+        ParticlesSet (the element-by-element set first, then one set per
+        BeamMonitor)."""
+        return ParticlesSet(self._output_buffer)
 
     @property
     def particles_buffer(self):
@@ -594,3 +597,198 @@ class TrackJob:
     @property
     def opencl_queue_addr(self):
         return 0
+
+
+class TrackJobBaseNew(TrackJob):
+    """The new-style job of the reference's Python binding
+    (python/sixtracklib/trackjob.py ``TrackJobBaseNew``, driving
+    NS(TrackJobNew_*)): built around an existing job pointer, then given its
+    buffers by ``_reset_detailed``.  The C job is the same object as the
+    legacy job's, so every TrackJob method applies; the reference's
+    new-style properties are added below."""
+
+    def __init__(self, ptr_track_job=None, owns_ptr=True):
+        self.ptr_st_track_job = ptr_track_job if ptr_track_job else None
+        self._owns_ptr = bool(owns_ptr)
+        self._c_buffers = []
+        self._last_status = 0
+        self._last_track_status = 0
+        self._arch = "hip"
+        self._output_buffer = None
+        self._particles_buffer = None
+        self._beam_elements_buffer = None
+        self._c_pbuf = self._c_ebuf = self._c_obuf = None
+        self._until_turn_ebe = 0
+        self._set_indices = (st.st_buffer_size_t * 1)(0)
+        cfg = (st.st_TrackJobNew_get_config_str(self.ptr_st_track_job)
+               if self.ptr_st_track_job else None)
+        self._config_str = cfg.decode() if cfg else ""
+
+    def __del__(self, _job_delete=st.st_TrackJob_delete):
+        if getattr(self, "ptr_st_track_job", None) and getattr(self, "_owns_ptr", False):
+            _job_delete(self.ptr_st_track_job)
+        self.ptr_st_track_job = None
+        if getattr(self, "_c_buffers", None):
+            self._free_buffers()
+
+    def _reset_detailed(self, beam_elements_buffer, particles_buffer, particle_set_index=0,
+                        until_turn_elem_by_elem=0, output_buffer=None):
+        """Give the job its buffers (one particle set of the particle
+        buffer); an output buffer is made when one is needed and none is
+        given."""
+        if not self.ptr_st_track_job:
+            raise ValueError("TrackJob has to be initialized before calling reset")
+        old = self._c_buffers
+        self._c_buffers = []
+        try:
+            eb = get_cbuffer_from_obj(beam_elements_buffer)
+            pb = get_cbuffer_from_obj(particles_buffer)
+            c_pbuf, c_ebuf = self._map(pb), self._map(eb)
+            if st.st_Particles_buffer_get_num_of_particle_blocks(c_pbuf) <= particle_set_index:
+                raise ValueError("the particle buffer has no particle set %d"
+                                 % particle_set_index)
+            idx = (st.st_buffer_size_t * 1)(int(particle_set_index))
+            ob, c_obuf = self._prepare_output(c_pbuf, idx, c_ebuf, int(until_turn_elem_by_elem),
+                                              output_buffer)
+            self._last_status = st.st_TrackJobNew_reset_detailed(
+                self.ptr_st_track_job, c_pbuf, 1, idx, c_ebuf, c_obuf,
+                int(until_turn_elem_by_elem))
+            if self._last_status != 0:
+                raise RuntimeError("unable to reset the TrackJob: " + st.last_error())
+        except Exception:
+            self._free_buffers()
+            self._c_buffers = old
+            raise
+        for p in old:
+            st.st_Buffer_delete(p)
+        self._beam_elements_buffer, self._particles_buffer = eb, pb
+        self._c_pbuf, self._c_ebuf = c_pbuf, c_ebuf
+        self._output_buffer, self._c_obuf = ob, c_obuf
+        self._until_turn_ebe = int(until_turn_elem_by_elem)
+        self._set_indices = idx
+        return self
+
+    def reset(self, beam_elements_buffer, particles_buffer, until_turn_elem_by_elem=0,
+              output_buffer=None):
+        return self._reset_detailed(beam_elements_buffer, particles_buffer, 0,
+                                    until_turn_elem_by_elem, output_buffer)
+
+    # -- the reference's new-style properties --------------------------------
+    @property
+    def pointer(self):
+        return self.ptr_st_track_job
+
+    @property
+    def last_status_success(self):
+        return self._last_status == 0
+
+    @property
+    def last_track_status_success(self):
+        return self._last_track_status == 0
+
+    @property
+    def uses_controller(self):
+        return bool(st.st_TrackJobNew_uses_controller(self.ptr_st_track_job))
+
+    @property
+    def uses_arguments(self):
+        return bool(st.st_TrackJobNew_uses_arguments(self.ptr_st_track_job))
+
+    @property
+    def is_in_debug_mode(self):
+        return bool(st.st_TrackJobNew_is_in_debug_mode(self.ptr_st_track_job))
+
+    @property
+    def is_collecting_particles(self):
+        return bool(st.st_TrackJobNew_is_collecting_particles(self.ptr_st_track_job))
+
+    @property
+    def is_collecting_beam_elements(self):
+        return bool(st.st_TrackJobNew_is_collecting_beam_elements(self.ptr_st_track_job))
+
+    @property
+    def is_collecting_output(self):
+        return bool(st.st_TrackJobNew_is_collecting_output(self.ptr_st_track_job))
+
+    @property
+    def collecting_flags(self):
+        return int(st.st_TrackJobNew_get_collect_flags(self.ptr_st_track_job))
+
+    @property
+    def num_particle_sets(self):
+        return int(st.st_TrackJobNew_get_num_particle_sets(self.ptr_st_track_job))
+
+    @property
+    def total_num_particles(self):
+        return int(st.st_TrackJobNew_get_total_num_of_particles(self.ptr_st_track_job))
+
+    @property
+    def has_config_str(self):
+        return bool(st.st_TrackJobNew_has_config_str(self.ptr_st_track_job))
+
+    @property
+    def owns_output_buffer(self):
+        return bool(st.st_TrackJobNew_owns_output_buffer(self.ptr_st_track_job))
+
+    @property
+    def has_beam_monitors(self):
+        return bool(st.st_TrackJobNew_has_beam_monitors(self.ptr_st_track_job))
+
+    @property
+    def beam_monitor_output_buffer_offset(self):
+        return int(st.st_TrackJobNew_get_beam_monitor_output_buffer_offset(self.ptr_st_track_job))
+
+    @property
+    def elem_by_elem_output_buffer_offset(self):
+        return int(st.st_TrackJobNew_get_elem_by_elem_output_buffer_offset(self.ptr_st_track_job))
+
+    @property
+    def num_elem_by_elem_turn(self):
+        return int(st.st_TrackJobNew_get_num_elem_by_elem_turns(self.ptr_st_track_job))
+
+    @property
+    def has_elem_by_elem_config(self):
+        return bool(st.st_TrackJobNew_has_elem_by_elem_config(self.ptr_st_track_job))
+
+    def collect(self, collect_flags=None):
+        if collect_flags is None:
+            st.st_TrackJobNew_collect(self.ptr_st_track_job)
+        elif collect_flags > 0:
+            st.st_TrackJobNew_collect_detailed(self.ptr_st_track_job, int(collect_flags))
+        return self
+
+    def collect_particle_addresses(self):
+        self._last_status = st.st_TrackJobNew_collect_particles_addresses(self.ptr_st_track_job)
+        return self
+
+    def collect_debug_flag(self):
+        self._last_status = st.st_TrackJobNew_collect_debug_flag(self.ptr_st_track_job)
+        return self
+
+    def enable_debug_mode(self):
+        self._last_status = st.st_TrackJobNew_enable_debug_mode(self.ptr_st_track_job)
+        if self._last_status != 0:
+            raise RuntimeError("unsuccessful enable debug mode op; status:%d" % self._last_status)
+        return self
+
+    def disable_debug_mode(self):
+        self._last_status = st.st_TrackJobNew_disable_debug_mode(self.ptr_st_track_job)
+        if self._last_status != 0:
+            raise RuntimeError("unsuccessful disable debug mode op; status:%d" % self._last_status)
+        return self
+
+    # the reference's deprecated camel-case spellings
+    def collectParticles(self):
+        return self.collect_particles()
+
+    def collectBeamElements(self):
+        return self.collect_beam_elements()
+
+    def collectOutput(self):
+        return self.collect_output()
+
+    def collectParticlesAddresses(self):
+        return self.collect_particle_addresses()
+
+    def collectDebugFlag(self):
+        return self.collect_debug_flag()

@@ -369,3 +369,52 @@ def test_new_style_and_cuda_job_without_device():
     assert not st.st_CudaTrackJob_get_ptr_particles_arg(cj)
     assert st.st_CudaTrackJob_default_threads_per_block(cj) == 256
     st.st_TrackJobNew_delete(cj)
+
+
+def test_python_buffer_handle(tmp_path):
+    """python/sixtracklib/buffer.py ``Buffer``: mapped on a CBuffer (the C
+    buffer sees the CBuffer's objects and capacities), read from a file,
+    made empty with given capacities, grown by reserve."""
+    from sixtracklib_amd.buffer import Buffer
+    eb = lattices.fodo_ring()
+    b = Buffer.from_cbuffer(eb)
+    assert b.maps_to_cbuffer and b.cbuffer is eb and not b.owns_buffer
+    assert b.num_objects == eb.n_objects == 402
+    assert b.slot_size == 8 and b.size == eb.words.nbytes
+    assert b.data_begin_addr == eb.pointer and not b.needs_remapping
+    assert b.max_num_objects >= 402 and b.max_num_slots >= b.num_slots
+    path = str(tmp_path / "fodo.bin")
+    assert b.write_to_file(path)
+    c = Buffer(path_to_file=path)
+    assert c.num_objects == 402 and c.is_owner and not c.maps_to_cbuffer
+    d = Buffer(num_objects=8, num_slots=64, num_dataptrs=4)
+    assert d.num_objects == 0 and d.max_num_objects >= 8 and d.max_num_slots >= 64
+    d.reserve(max_num_objects=100, max_num_slots=4000, max_num_dataptrs=40)
+    assert d.max_num_objects >= 100 and d.max_num_slots >= 4000 and d.max_num_dataptrs >= 40
+    assert Buffer(size=1024).num_objects == 0
+
+
+def test_particles_set_lists_the_buffer_and_particles_addr():
+    from sixtracklib_amd.particles import ParticlesSet, ParticlesAddr, particles_in
+    from sixtracklib_amd.config_helper import supports
+    ps = ParticlesSet()
+    ps.Particles(num_particles=3)
+    ps.Particles(num_particles=5)
+    again = ParticlesSet(ps.cbuffer)
+    assert [p.num_particles for p in again.particles] == [3, 5] and again.num_particles == 8
+    assert particles_in(again, 1).num_particles == 5
+    cb = CBuffer()
+    a = ParticlesAddr(cbuffer=cb, num_particles=7, x=0x1000, state=0x2000)
+    assert cb.get_object_typeid(0) == 512 and a.x == 0x1000 and a.state == 0x2000 and a.px == 0
+    assert supports("hip") and supports("cuda") and not supports("cpu")
+
+
+def test_cuda_trackjob_without_device_raises():
+    from sixtracklib_amd.cuda import CudaTrackJob
+    from sixtracklib_amd import stcommon as st
+    if st.st_Hip_num_devices() > 0:
+        pytest.skip("a HIP device is present")
+    eb = lattices.fodo_ring()
+    pb, _ = lattices.gaussian_beam(10, seed=0)
+    with pytest.raises(RuntimeError):
+        CudaTrackJob(eb, pb)

@@ -221,3 +221,50 @@ def test_python_trackjob_reference_methods():
     assert job.index_of_assign_address_item(item) == 0
     assert job.ptr_assign_address_item(dest_buffer_id=2, src_buffer_id=1024, index=0)
     del job
+
+
+def test_python_cuda_trackjob_matches_oracle():
+    """The reference's Python GPU job (python/sixtracklib/cuda.py
+    ``CudaTrackJob``) on HIP: node "0.0", the exact build; track_until and
+    element by element into its output ParticlesSet, bit-identical to the
+    oracle; the new-style properties and camel-case spellings."""
+    from sixtracklib_amd import CudaTrackJob, TrackJobBaseNew
+    eb = lattices.fodo_ring()
+    pb = _beam()
+    ob = _copy(pb)
+    assert oracle_lib.track_until(ob, eb, 7) == 0
+    job = CudaTrackJob(eb, pb, node_id_str="0.0", config_str="fma=0")
+    assert isinstance(job, TrackJobBaseNew)
+    assert not job.has_controller and not job.has_particles_arg and not job.uses_controller
+    assert job.num_particle_sets == 1 and job.total_num_particles == 777
+    assert job.default_threads_per_block > 0
+    assert not job.is_in_debug_mode
+    assert job.enable_debug_mode().is_in_debug_mode
+    job.disable_debug_mode()
+    job.track_until(7).collectParticles()
+    assert job.last_track_status_success
+    assert_particles_equal(_pdict(pb), _pdict(ob), what="CudaTrackJob")
+    # element by element: the job's output ParticlesSet holds the oracle's store
+    eb2 = lattices.fodo_ring()
+    pb2 = lattices.gaussian_beam(64, seed=13, sigma_x=4e-4, sigma_y=4e-4, sigma_px=4e-6,
+                                 sigma_py=4e-6)[0]
+    job2 = CudaTrackJob(eb2, pb2, until_turn_elem_by_elem=2, config_str="fma=0")
+    assert job2.has_elem_by_elem_output and job2.has_elem_by_elem_config
+    job2.track_elem_by_elem(2)
+    job2.collect()
+    out = job2.output
+    off = job2.elem_by_elem_output_buffer_offset
+    n_out = out.particles[off].num_particles
+    assert n_out == 64 * 402 * 2
+    ocb = CBuffer(max_slots=22 * n_out + 64)
+    Particles(cbuffer=ocb, num_particles=n_out)
+    ob2 = _copy(lattices.gaussian_beam(64, seed=13, sigma_x=4e-4, sigma_y=4e-4, sigma_px=4e-6,
+                                       sigma_py=4e-6)[0])
+    cfg = oracle_lib.EbeConfig(order=0, num_particles_to_store=64, num_elements_to_store=402,
+                               num_turns_to_store=2, min_particle_id=0, min_element_id=0,
+                               min_turn=0, max_particle_id=63, max_element_id=401, max_turn=1,
+                               is_rolling=1, out_store_addr=ocb.get_object_address(0))
+    assert oracle_lib.track_elem_by_elem(ob2, eb2, cfg, 2) == 0
+    assert_particles_equal(_pdict(out.cbuffer, off), _pdict(ocb), what="CudaTrackJob ebe store")
+    assert_particles_equal(_pdict(pb2), _pdict(ob2), what="CudaTrackJob ebe particles")
+    del job, job2
