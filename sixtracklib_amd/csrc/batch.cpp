@@ -284,21 +284,25 @@ struct NS(TrackBatch)
                 for( int q : { 1, 8 } )
                 {
                     Config c = base;
-                    c.ppl = 1; c.staging = 0; c.block = b; c.waves_per_cu = w; c.queues = q;
+                    c.ppl = 1; c.ppl_auto = false; c.staging = 0; c.block = b; c.waves_per_cu = w;
+                    c.queues = q;
                     cand.push_back( c );
                 }
-        for( int q : { 1, 8 } )
-        {
-            Config c = base;
-            c.ppl = 2; c.staging = 0; c.block = 256; c.waves_per_cu = 20; c.queues = q;
-            cand.push_back( c );
-        }
+        for( int w : { 16, 32 } )
+            for( int q : { 1, 8 } )
+            {
+                Config c = base;
+                c.ppl = 2; c.ppl_auto = false; c.staging = 0; c.block = 256; c.waves_per_cu = w;
+                c.queues = q;
+                cand.push_back( c );
+            }
         if( run.max_record_words + 2 <= ( int64_t )ST_LAT_STAGE_WORDS )
             for( int w : { 16, 32 } )
                 for( int q : { 1, 8 } )
                 {
                     Config c = base;
-                    c.ppl = 1; c.staging = 1; c.block = 256; c.waves_per_cu = w; c.queues = q;
+                    c.ppl = 1; c.ppl_auto = false; c.staging = 1; c.block = 256; c.waves_per_cu = w;
+                    c.queues = q;
                     cand.push_back( c );
                 }
         std::vector< std::pair< float, Config > > timed;

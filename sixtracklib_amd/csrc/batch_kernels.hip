@@ -326,14 +326,12 @@ KN( st_k_batch_track_lds )( st_batch_sched_t s, int64_t until_turn, int64_t max_
     }
 }
 
-// Occupancy target of the two-particles-per-lane track kernel: its two
-// state sets fit 96 VGPRs, 5 waves per SIMD.
-#ifndef ST_LEAN2_WAVES_PER_EU
-#define ST_LEAN2_WAVES_PER_EU 5
-#endif
-
-// As st_k_batch_track with 128-particle tiles, two particles per lane.
-__global__ void __launch_bounds__( 256, ST_LEAN2_WAVES_PER_EU )
+// As st_k_batch_track with 128-particle tiles, two particles per lane, at
+// the same eight waves per SIMD (track_lean_lane2 keeps the pair in 64
+// VGPRs; its rare maps spill a few registers to scratch).  The particle set
+// is read from the seed descriptor where it is used, so that no field
+// address stays in SGPRs across the element loop.
+__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
 KN( st_k_batch_track2 )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
                          int32_t* status )
 {
@@ -355,9 +353,9 @@ KN( st_k_batch_track2 )( st_batch_sched_t s, int64_t until_turn, int64_t max_tur
             int32_t count = 0;
             const int32_t* list = nullptr;
             if( !find_tile< 128 >( s, qb, qe, t, seed, list, count, tile ) ) break;  // queue drained
-            SeedView const S = load_seed( s.seeds, seed );
-            track_lean_lane2( S.P, list, count, ( int64_t )tile * 128, S.lat, S.n_elem, until_turn,
-                              max_turns, status, wp );
+            cptr_t const w = ( cptr_t )s.seeds + 25 * seed;
+            track_lean_lane2( PDesc{ w }, list, count, ( int64_t )tile * 128, ( cptr_t )w[ 22 ],
+                              ( int64_t )w[ 23 ], until_turn, max_turns, status, wp );
             if( one_tile ) return;
         }
     }

@@ -60,6 +60,9 @@ struct Config
     int64_t turns_per_launch = 0;
     bool graph = true;
     int ppl = 1;                 // particles per lane (1 or 2)
+    bool ppl_auto = false;       // balanced form: per request, 2 when the particles
+                                 // fill at least 85 % of the resident waves with
+                                 // 128-particle tiles, else 1 (see run_balanced)
     bool heavy = false;          // lattices hold maps outside the lean set
                                  // (set by the owner, one particle per lane)
     bool balance = false;        // run_balanced() available (split lists)
@@ -72,9 +75,10 @@ struct Config
     {
         char b[ 224 ];
         std::snprintf( b, sizeof( b ), "fma=%d block=%d waves_per_cu=%d queues=%d "
-                       "turns_per_launch=%lld graph=%d ppl=%d wave_prio=%d staging=%s%s%s",
+                       "turns_per_launch=%lld graph=%d ppl=%s wave_prio=%d staging=%s%s%s",
                        fma ? 1 : 0, block, waves_per_cu, queues, ( long long )turns_per_launch,
-                       graph ? 1 : 0, ppl, wave_prio ? 1 : 0, staging ? "lds" : "scalar",
+                       graph ? 1 : 0, ppl_auto ? "auto" : ( ppl == 2 ? "2" : "1" ),
+                       wave_prio ? 1 : 0, staging ? "lds" : "scalar",
                        heavy ? " heavy" : "", balance ? " balance" : "" );
         return b;
     }
@@ -105,8 +109,10 @@ struct Config
         else if( k == "balance" ) balance = iv != 0;
         else if( k == "ppl" )
         {
-            if( iv != 1 && iv != 2 ) return fail( "ppl (particles per lane) must be 1 or 2" );
+            if( v == "auto" ) { ppl_auto = true; ppl = 1; return true; }
+            if( iv != 1 && iv != 2 ) return fail( "ppl (particles per lane) must be 1, 2 or auto" );
             ppl = ( int )iv;
+            ppl_auto = false;
         }
         else if( k == "staging" )
         {
@@ -533,6 +539,15 @@ struct BatchRunner
         return hip_ok( e, "launch batch track" );
     }
 
+    // tiles of the split lists read back by select_split_sync
+    int64_t split_tiles() const
+    {
+        int64_t const tile = cfg.tile_particles();
+        int64_t t = 0;
+        for( int64_t q = 0; q < 2 * n_seeds(); ++q ) t += ( h_split[ q ] + tile - 1 ) / tile;
+        return t;
+    }
+
     // Balanced form, one select pass read back (synchronises the stream):
     // particles with turns to go, the smallest at_turn, and the tiles of
     // the split lists.
@@ -577,6 +592,16 @@ struct BatchRunner
             if( n_active == 0 ) return true;
             if( group == 0 )
             {
+                // ppl=auto: two particles per lane run the record walk once for
+                // both (7 % more particle-turns per second with every resident
+                // wave busy), but hold half the waves of one per lane; they
+                // are chosen when the particles fill at least 85 % of the
+                // resident waves with 128-particle tiles
+                if( cfg.ppl_auto && !cfg.heavy && !cfg.lds() )
+                {
+                    cfg.ppl = ( 100 * n_active >= 85 * 128 * W ) ? 2 : 1;
+                    tiles = split_tiles();
+                }
                 k = balanced_turns( tiles, until_turn - min_turn, n_active );
                 limit = 16 + 4 * ( ( tiles + W - 1 ) / W ) * ( ( until_turn - min_turn + k - 1 ) / k );
             }

@@ -224,7 +224,7 @@ struct NS(TrackJobBase)
     // one tile queue: a job's sets share one lattice, and with a queue per
     // XCD the waves of the first XCD would take every tile of a small set
     int queues = 1;
-    int ppl = 1;                         // particles per lane of the lean until path
+    int ppl = 0;                         // particles per lane of the lean until path (0: auto)
     bool balance = true;                 // one tile per wave and launch, furthest behind first
     bool wave_prio = true;               // waves with more turns left issue first
     int staging = 0;                     // lean until path: lattice via scalar loads or LDS
@@ -639,13 +639,14 @@ struct NS(TrackJobBase)
         c.turns_per_launch = turns_per_launch;
         c.queues = queues;
         c.heavy = lat_full;
-        c.ppl = lat_full ? 1 : ppl;
+        c.ppl = ( lat_full || ppl != 2 ) ? 1 : 2;
+        c.ppl_auto = !lat_full && ppl == 0 && staging == 0;
         c.balance = balance;
         c.wave_prio = wave_prio;
         c.staging = ( lat_full || c.ppl == 2 ) ? 0 : staging;
-        // resident waves per CU: 8 per SIMD with one particle per lane, 5
-        // with two (the kernels' register budgets)
-        c.waves_per_cu = c.ppl == 2 ? 20 : 32;
+        // resident waves per CU: 8 per SIMD (the lean kernels' register
+        // budget, with one particle per lane and with two)
+        c.waves_per_cu = 32;
         if( waves_per_cu > 0 ) c.waves_per_cu = std::min( waves_per_cu, c.waves_per_cu );
         c.graph = false;
         return c;
@@ -841,7 +842,7 @@ struct NS(TrackJobBase)
             else if( k == "compact" ) compact = iv != 0;
             else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
             else if( k == "queues" ) queues = ( iv == 8 ) ? 8 : 1;
-            else if( k == "ppl" ) ppl = ( iv == 2 ) ? 2 : 1;
+            else if( k == "ppl" ) ppl = ( v == "auto" ) ? 0 : ( iv == 2 ) ? 2 : 1;
             else if( k == "balance" ) balance = iv != 0;
             else if( k == "wave_prio" ) wave_prio = iv != 0;
             else if( k == "staging" ) staging = ( v == "lds" || v == "1" ) ? 1 : 0;

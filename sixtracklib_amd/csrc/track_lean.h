@@ -313,49 +313,185 @@ ST_DEV void track_lean_lane( const PA& pa, const int32_t* list, int64_t n_lanes,
     if( in_range && status != 0 ) atomicOr( status_out, status );
 }
 
+// Store of a finished lane of the hot-state loop: delta and psigma are not
+// held in registers there (memory holds their current values), s is the
+// given one.
+ST_DEV void store_hot_state( const st_dev_particles_t& P, int64_t i, const Lane& L, double s,
+                             int64_t state, int64_t at_element, int64_t at_turn )
+{
+    int64_t j = i;
+    asm volatile( "" : "+v"( j ) );
+    P.x[ j ] = L.x; P.px[ j ] = L.px; P.y[ j ] = L.y; P.py[ j ] = L.py;
+    P.zeta[ j ] = L.zeta; P.rpp[ j ] = L.rpp; P.rvv[ j ] = L.rvv; P.s[ j ] = s;
+    P.state[ j ] = state;
+    P.at_element[ j ] = at_element;
+    P.at_turn[ j ] = at_turn;
+}
+
+// Records other than Drift / Multipole for the hot-state loop: the maps
+// that read delta (Cavity, DriftExact) take it from memory, and Cavity
+// writes delta and psigma back.
+template< class PA >
+ST_DEV bool lean_other_hot( unsigned op, cptr_t e, Lane& L, const PA& pa, int64_t i,
+                            bool active, bool& fail )
+{
+    switch( op )
+    {
+    case ST_OP_DRIFT_EXACT:
+    {
+        int64_t j = i;
+        asm volatile( "" : "+v"( j ) );
+        auto const& P = pa.view();
+        L.delta = P.delta[ j ];
+        L.s = 0.0;
+        drift_exact( L, rd( e, 0 ) );
+        return limit_global_ok( L );
+    }
+    case ST_OP_CAVITY:
+    {
+        int64_t j = i;
+        asm volatile( "" : "+v"( j ) );
+        auto const& P = pa.view();
+        LaneConst C;
+        C.q0 = P.q0[ j ]; C.mass0 = P.mass0[ j ]; C.beta0 = P.beta0[ j ];
+        C.p0c = P.p0c[ j ]; C.chi = P.chi[ j ]; C.charge_ratio = P.charge_ratio[ j ];
+        L.delta = P.delta[ j ];
+        cavity( L, C, e );
+        if( active ) { P.delta[ j ] = L.delta; P.psigma[ j ] = L.psigma; }
+        return true;
+    }
+    case ST_OP_XYSHIFT: xyshift( L, e ); return true;
+    case ST_OP_SROTATION: srotation( L, e ); return true;
+    case ST_OP_LIMIT_RECT: return limit_rect_ok( L, e );
+    case ST_OP_LIMIT_ELLIPSE: return limit_ellipse_ok( L, e );
+    case ST_OP_LIMIT_RECT_ELLIPSE: return limit_rect_ellipse_ok( L, e );
+    case ST_OP_DIPEDGE: dipedge( L, e ); return true;
+    default: break;
+    }
+    fail = true;
+    return false;
+}
+
+// Horner recurrence of two particles through one record: the coefficient
+// reads and the order dispatch are shared, the two chains interleave.
+template< int N >
+ST_DEV void horner_fixed2( cptr_t e, double const x0, double const y0, double const x1,
+                           double const y1, double& dpx0, double& dpy0, double& dpx1, double& dpy1 )
+{
+    dpx0 = dpx1 = rd( e, 2 * N );
+    dpy0 = dpy1 = rd( e, 2 * N + 1 );
+#pragma unroll
+    for( int k = N; k > 0; --k )
+    {
+        double const bre = rd( e, 2 * k - 2 ), bim = rd( e, 2 * k - 1 );
+        horner_step( dpx0, dpy0, x0, y0, bre, bim );
+        horner_step( dpx1, dpy1, x1, y1, bre, bim );
+    }
+}
+
+// be_multipole/track.h:38-112 for two particles without curvature terms
+// (the caller runs multipole() for records with hxl or hyl != 0 and for
+// orders above 10).
+ST_DEV void multipole2_flat( Lane& p0, Lane& p1, double const chi0, double const chi1, cptr_t e,
+                             int const order )
+{
+    double dpx0, dpy0, dpx1, dpy1;
+    double const x0 = p0.x, y0 = p0.y, x1 = p1.x, y1 = p1.y;
+    switch( order )
+    {
+    case 0: horner_fixed2< 0 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 1: horner_fixed2< 1 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 2: horner_fixed2< 2 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 3: horner_fixed2< 3 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 4: horner_fixed2< 4 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 5: horner_fixed2< 5 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 6: horner_fixed2< 6 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 7: horner_fixed2< 7 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 8: horner_fixed2< 8 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    case 9: horner_fixed2< 9 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    default: horner_fixed2< 10 >( e, x0, y0, x1, y1, dpx0, dpy0, dpx1, dpy1 ); break;
+    }
+#if defined( ST_FMA_BUILD )
+    p0.px = fma( -chi0, dpx0, p0.px );
+    p0.py = fma( chi0, dpy0, p0.py );
+    p1.px = fma( -chi1, dpx1, p1.px );
+    p1.py = fma( chi1, dpy1, p1.py );
+#else
+    dpx0 = -chi0 * dpx0;
+    dpy0 = chi0 * dpy0;
+    dpx1 = -chi1 * dpx1;
+    dpy1 = chi1 * dpy1;
+    p0.px += dpx0;
+    p0.py += dpy0;
+    p1.px += dpx1;
+    p1.py += dpy1;
+#endif
+}
+
 // Two particles per lane: lane l of a wave tracks the particles of slots
 // base + l and base + 64 + l (list entries, or indices without a list)
-// through the same records.  Each particle's map sequence, loss handling
-// and stored values are exactly those of track_lean_lane; what the pair
-// shares is the record walk (scalar loads, header tests, branches), and
-// the two state sets give the scheduler two independent dependency chains
-// per lane (the Horner recurrences in particular).
-ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, int64_t n_lanes,
-                              int64_t base, cptr_t lat, int64_t n_elem, int64_t until_turn,
-                              int64_t max_turns, int32_t* status_out, const WavePrio& wp = WavePrio() )
+// through the same records.  Each particle's maps, loss handling and stored
+// values are those of track_lean_lane, bit for bit; what the pair shares is
+// the record walk (scalar loads, header tests, branches, a multipole's order
+// dispatch and coefficient reads), and the two Horner chains of a multipole
+// interleave.  Only the state the Drift and Multipole records use stays in
+// registers (x, px, y, py, zeta, rpp, rvv and chi: 16 VGPRs a particle), so
+// that the pair fits the 64 VGPRs of eight waves per SIMD:
+//   * delta and psigma stay in memory: Cavity and DriftExact read delta
+//     there, Cavity writes both back, and a store leaves them as they are;
+//   * every turn starts at s = +0 and s after a drift is the record's, as
+//     in lean_turn's turns without a per-lane s;
+//   * at_element and at_turn are read back at a store (memory holds their
+//     values at the start of the launch until then); the turns a lane has
+//     left are a 32-bit count against the wave's turn counter.
+// A tile with a particle that starts its turn at s != 0 runs through
+// track_lean_lane, one half after the other.
+template< class PA >
+ST_DEV void track_lean_lane2( const PA& pa, const int32_t* list, int64_t n_lanes,
+                               int64_t base, cptr_t lat, int64_t n_elem, int64_t until_turn,
+                               int64_t max_turns, int32_t* status_out, const WavePrio& wp = WavePrio() )
 {
     constexpr int K = 2;
-    int64_t i[ K ], at_turn[ K ], start[ K ], turn_stop[ K ];
+    int32_t idx[ K ], left[ K ];               // particle index, turns this launch
     double chi[ K ];
     Lane L[ K ];
     uint64_t act[ K ];
-    int32_t status = 0;
+    bool mid = false;                          // a particle starts at s != 0
 #pragma unroll
     for( int k = 0; k < K; ++k )
     {
+        auto const& P = pa.view();
         int64_t const tid = base + 64 * k + __lane_id();
         bool const in_range = tid < n_lanes;
-        i[ k ] = ( list != nullptr && in_range ) ? ( int64_t )list[ tid ] : tid;
-        int64_t const li = in_range ? i[ k ] : 0;
+        int64_t const i = ( list != nullptr && in_range ) ? ( int64_t )list[ tid ] : tid;
+        int64_t const li = in_range ? i : 0;
+        idx[ k ] = ( int32_t )li;
         L[ k ].x = P.x[ li ]; L[ k ].px = P.px[ li ]; L[ k ].y = P.y[ li ];
-        L[ k ].py = P.py[ li ]; L[ k ].zeta = P.zeta[ li ]; L[ k ].delta = P.delta[ li ];
-        L[ k ].rpp = P.rpp[ li ]; L[ k ].rvv = P.rvv[ li ]; L[ k ].psigma = P.psigma[ li ];
-        L[ k ].s = P.s[ li ];
+        L[ k ].py = P.py[ li ]; L[ k ].zeta = P.zeta[ li ];
+        L[ k ].rpp = P.rpp[ li ]; L[ k ].rvv = P.rvv[ li ];
         chi[ k ] = P.chi[ li ];
-        start[ k ] = P.at_element[ li ];
-        at_turn[ k ] = P.at_turn[ li ];
-        bool const active0 = in_range && n_elem > 0 && until_turn > at_turn[ k ] &&
-                             P.state[ li ] == 1;
-        if( in_range && n_elem <= 0 ) status = -1;
-        turn_stop[ k ] = ( max_turns > 0 && until_turn - at_turn[ k ] > max_turns )
-                             ? at_turn[ k ] + max_turns : until_turn;
+        int64_t const at_turn = P.at_turn[ li ];
+        bool const active0 = in_range && n_elem > 0 && until_turn > at_turn && P.state[ li ] == 1;
+        int64_t const stop = ( max_turns > 0 && until_turn - at_turn > max_turns ) ? at_turn + max_turns
+                                                                                     : until_turn;
+        left[ k ] = active0 ? ( int32_t )( stop - at_turn < INT32_MAX ? stop - at_turn : INT32_MAX ) : 0;
+        mid = mid || ( active0 && __double_as_longlong( P.s[ li ] ) != 0 );
         act[ k ] = ballot( active0 );
     }
-
-    int64_t turns_done = 0;
+    if( ballot( mid ) != 0u || n_elem <= 0 )
+    {
+        track_lean_lane< false >( pa, list, n_lanes, base + __lane_id(), lat, n_elem,
+                                  until_turn, max_turns, status_out, wp );
+        track_lean_lane< false >( pa, list, n_lanes, base + 64 + __lane_id(), lat, n_elem,
+                                  until_turn, max_turns, status_out, WavePrio() );
+        return;
+    }
+    int32_t status = 0;
+    int64_t turns_done = 0;                    // by this wave, in this call
     wave_priority( wp, 0 );
     while( ( act[ 0 ] | act[ 1 ] ) != 0u )
     {
+        uint64_t s_rec = 0;                    // s after the last drift (bits)
         uint32_t off = 0;
         uint64_t h = rec_at( lat, 0 )[ 0 ];
         for( ;; )
@@ -367,28 +503,51 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
                 uint64_t const hn = rec_at( lat, noff )[ 0 ];
                 if( ( uint32_t )h & ST_OP_MP_BIT )
                 {
+                    unsigned const aux = ST_REC_AUX( h );
+                    if( ( aux & ST_MP_CURV_FLAG ) == 0u && ( aux & 0x7fffu ) <= 10u )
+                    {                          // the common record: one dispatch for both
+                        multipole2_flat( L[ 0 ], L[ 1 ], chi[ 0 ], chi[ 1 ], r + ST_REC_PAYLOAD,
+                                         ( int )( aux & 0x7fffu ) );
+                    }
+                    else if( aux & ST_MP_CURV_FLAG )
+                    {
 #pragma unroll
-                    for( int k = 0; k < K; ++k )
-                        multipole( L[ k ], chi[ k ], r + ST_REC_PAYLOAD, ST_REC_AUX( h ),
-                                   L[ k ].delta );
+                        for( int k = 0; k < K; ++k )
+                        {
+                            int64_t j = idx[ k ];
+                            asm volatile( "" : "+v"( j ) );
+                            multipole( L[ k ], chi[ k ], r + ST_REC_PAYLOAD, aux, pa.view().delta[ j ] );
+                        }
+                    }
+                    else
+                    {
+#pragma unroll
+                        for( int k = 0; k < K; ++k )
+                            multipole( L[ k ], chi[ k ], r + ST_REC_PAYLOAD, aux, 0.0 );
+                    }
                 }
                 else
                 {
-                    double const length = rd( r, ST_REC_PAYLOAD );
+                    double const length = rd( r, 2 );
+                    s_rec = r[ 3 ];
+#pragma unroll
+                    for( int k = 0; k < K; ++k ) drift_xz( L[ k ], length );
+                    uint64_t const lost0 = limit_global_lost_mask( L[ 0 ] ) & act[ 0 ];
+                    uint64_t const lost1 = limit_global_lost_mask( L[ 1 ] ) & act[ 1 ];
+                    if( ( lost0 | lost1 ) != 0u )
 #pragma unroll
                     for( int k = 0; k < K; ++k )
                     {
-                        drift( L[ k ], length );
-                    }
-#pragma unroll
-                    for( int k = 0; k < K; ++k )
-                    {
-                        uint64_t const lost = limit_global_lost_mask( L[ k ] ) & act[ k ];
+                        uint64_t const lost = k == 0 ? lost0 : lost1;
                         if( lost != 0u )
                         {
                             if( lane_in( lost ) )
-                                store_lane_state( P, i[ k ], L[ k ], 0, start[ k ] + ri( r, 1 ),
-                                                  at_turn[ k ] );
+                            {
+                                int64_t const j = idx[ k ];
+                                auto const& P = pa.view();
+                                store_hot_state( P, j, L[ k ], __longlong_as_double( ( long long )s_rec ), 0,
+                                                 P.at_element[ j ] + ri( r, 1 ), P.at_turn[ j ] + turns_done );
+                            }
                             act[ k ] &= ~lost;
                         }
                     }
@@ -401,21 +560,22 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
             cptr_t const r = rec_at( lat, off );
             uint32_t const noff = off + ST_REC_BYTES( h );
             uint64_t const hn = rec_at( lat, noff )[ 0 ];
+            if( op == ST_OP_DRIFT_EXACT ) s_rec = r[ 3 ];
 #pragma unroll
             for( int k = 0; k < K; ++k )
             {
-                int64_t const li = lane_in( act[ k ] ) ? i[ k ] : 0;
                 bool fail = false;
-                bool const ok = lean_other< false >( op, ST_REC_AUX( h ), r + ST_REC_PAYLOAD, L[ k ],
-                                                     PDirect{ P }, li, lane_in( act[ k ] ),
-                                                     start[ k ] + ri( r, 1 ), at_turn[ k ], fail );
+                bool const ok = lean_other_hot( op, r + ST_REC_PAYLOAD, L[ k ], pa, idx[ k ],
+                                                lane_in( act[ k ] ), fail );
                 uint64_t const lost = ballot( !ok ) & act[ k ];
                 if( lost != 0u )
                 {
                     if( lane_in( lost ) )
                     {
-                        store_lane_state( P, i[ k ], L[ k ], 0, start[ k ] + ri( r, 1 ),
-                                          at_turn[ k ] );
+                        int64_t const j = idx[ k ];
+                        auto const& P = pa.view();
+                        store_hot_state( P, j, L[ k ], __longlong_as_double( ( long long )s_rec ), 0,
+                                         P.at_element[ j ] + ri( r, 1 ), P.at_turn[ j ] + turns_done );
                         if( fail ) status = -1;
                     }
                     act[ k ] &= ~lost;
@@ -424,19 +584,22 @@ ST_DEV void track_lean_lane2( const st_dev_particles_t& P, const int32_t* list, 
             off = noff;
             h = hn;
         }
+        ++turns_done;
 #pragma unroll
         for( int k = 0; k < K; ++k )
         {
-            if( lane_in( act[ k ] ) ) { ++at_turn[ k ]; L[ k ].s = 0.0; }
-            uint64_t const fin = ballot( at_turn[ k ] >= turn_stop[ k ] ) & act[ k ];
+            uint64_t const fin = ballot( turns_done >= left[ k ] ) & act[ k ];
             if( fin != 0u )
             {
-                if( lane_in( fin ) ) store_lane_state( P, i[ k ], L[ k ], 1, start[ k ],
-                                                       at_turn[ k ] );
+                if( lane_in( fin ) )
+                {
+                    int64_t const j = idx[ k ];
+                    auto const& P = pa.view();
+                    store_hot_state( P, j, L[ k ], 0.0, 1, P.at_element[ j ], P.at_turn[ j ] + turns_done );
+                }
                 act[ k ] &= ~fin;
             }
         }
-        ++turns_done;
         if( ( act[ 0 ] | act[ 1 ] ) != 0u ) wave_priority( wp, turns_done );
     }
     wave_priority_leave( wp, turns_done );

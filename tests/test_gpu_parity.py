@@ -234,6 +234,64 @@ def test_two_particles_per_lane_match_one(fma):
         assert_particles_equal(res["two"], _pdict(ob), what="pairs vs oracle")
 
 
+def test_two_particles_per_lane_after_track_line_match_oracle():
+    """Particles that start a track_until mid-lattice (s != 0 after a
+    track_line without finishing the turn) run through the one-particle
+    loop inside the two-particle kernel; every third particle also starts
+    at a later turn.  Exact build against the one-particle kernel and the
+    oracle, bit for bit."""
+    from sixtracklib_amd.trackjob import TrackJob
+    from sixtracklib_amd.particles import particles_in
+    eb = lattices.lhc_like_ring(seed=3, n_cells=8, tune=8.31 * 8 / 40)
+    sig = lattices.twiss_sigma(beta=236.0)
+    pb, _ = lattices.polar_grid(40, 64, sigma_x=sig, sigma_y=sig, max_sigma=80.0,
+                                delta=2.7e-4)
+    p = particles_in(pb)
+    at = np.array(p.at_turn)
+    at[::3] = 2
+    p.at_turn[:] = at
+    bufs = {"one": _copy(pb), "two": _copy(pb), "oracle": _copy(pb)}
+    for key, cfg in (("one", "ppl=1"), ("two", "ppl=2")):
+        job = TrackJob(eb, bufs[key], config_str="fma=0 " + cfg)
+        job.track_line(0, 137, False)
+        assert job.track_until(6).last_track_status == 0
+        job.collect()
+    oracle_lib.track_line(bufs["oracle"], eb, 0, 137, False)
+    oracle_lib.track_until(bufs["oracle"], eb, 6)
+    one, two = _pdict(bufs["one"]), _pdict(bufs["two"])
+    assert 50 < np.count_nonzero(one["state"] == 0) < 2500
+    assert_particles_equal(two, one, what="pairs vs single after track_line")
+    assert_particles_equal(two, _pdict(bufs["oracle"]), what="pairs vs oracle after track_line")
+
+
+@pytest.mark.parametrize("fma", [False, True])
+def test_ppl_auto_matches_one_particle_per_lane(fma):
+    """ppl=auto (the default) takes two particles per lane for a request
+    whose particles fill at least 85 % of the resident waves with
+    128-particle tiles (27,853 with waves_per_cu=1, 256 resident waves).
+    40,000 particles to 200 sigma start above it and fall below it in the
+    first request (8,403 left after 3 turns); 35,840 to 20 sigma stay above
+    it for the three requests.  Results equal the one-particle kernel's bit
+    for bit."""
+    from sixtracklib_amd.trackjob import TrackJob
+    eb = lattices.lhc_like_ring(seed=3, n_cells=8, tune=8.31 * 8 / 40)
+    sig = lattices.twiss_sigma(beta=236.0)
+    for n_amp, max_sigma in ((625, 200.0), (560, 20.0)):
+        pb, _ = lattices.polar_grid(n_amp, 64, sigma_x=sig, sigma_y=sig, max_sigma=max_sigma,
+                                    delta=2.7e-4)
+        res = {}
+        for key, cfg in (("auto", ""), ("one", " ppl=1")):
+            buf = _copy(pb)
+            job = TrackJob(eb, buf, config_str="fma=%d waves_per_cu=1%s" % (int(fma), cfg))
+            for t in (3, 7, 12):
+                assert job.track_until(t).last_track_status == 0
+            job.collect()
+            res[key] = _pdict(buf)
+        if max_sigma > 100:
+            assert np.count_nonzero(res["one"]["state"] == 0) > 30000
+        assert_particles_equal(res["auto"], res["one"], what="auto vs ppl=1, %d particles" % (64 * n_amp))
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_kitchen_sink_exact_bitwise(seed):
     eb = lattices.kitchen_sink(seed)
