@@ -35,7 +35,7 @@
  * 1: one work queue for the persistent launch; 8: one per XCD),
  * "ppl=1|2|auto" (default auto: particles per lane of the until-turn kernel
  * of lean lattices; auto takes two for a request whose particles fill at
- * least 85 % of the resident waves with 128-particle tiles and one
+ * least 70 % of the resident waves with 128-particle tiles and one
  * otherwise; results identical), "balance=0|1" (default 1: with compact=1
  * every launch gives each resident wave one 64-particle tile, the
  * particles furthest behind first, and the host reads the counts back

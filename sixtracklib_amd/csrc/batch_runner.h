@@ -61,7 +61,7 @@ struct Config
     bool graph = true;
     int ppl = 1;                 // particles per lane (1 or 2)
     bool ppl_auto = false;       // balanced form: per request, 2 when the particles
-                                 // fill at least 85 % of the resident waves with
+                                 // fill at least 70 % of the resident waves with
                                  // 128-particle tiles, else 1 (see run_balanced)
     bool heavy = false;          // lattices hold maps outside the lean set
                                  // (set by the owner, one particle per lane)
@@ -593,13 +593,14 @@ struct BatchRunner
             if( group == 0 )
             {
                 // ppl=auto: two particles per lane run the record walk once for
-                // both (7 % more particle-turns per second with every resident
-                // wave busy), but hold half the waves of one per lane; they
-                // are chosen when the particles fill at least 85 % of the
+                // both (8 % more particle-turns per second on the 20,000-element
+                // ring with 75 % or more of the resident waves busy), but hold
+                // half the waves of one per lane (at 65 % they are 4 % slower);
+                // they are chosen when the particles fill at least 70 % of the
                 // resident waves with 128-particle tiles
                 if( cfg.ppl_auto && !cfg.heavy && !cfg.lds() )
                 {
-                    cfg.ppl = ( 100 * n_active >= 85 * 128 * W ) ? 2 : 1;
+                    cfg.ppl = ( 100 * n_active >= 70 * 128 * W ) ? 2 : 1;
                     tiles = split_tiles();
                 }
                 k = balanced_turns( tiles, until_turn - min_turn, n_active );

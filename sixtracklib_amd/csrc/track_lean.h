@@ -486,7 +486,7 @@ ST_DEV void track_lean_lane2( const PA& pa, const int32_t* list, int64_t n_lanes
                                   until_turn, max_turns, status_out, WavePrio() );
         return;
     }
-    int32_t status = 0;
+    bool failed = false;                       // a lane met a disabled / unknown map
     int64_t turns_done = 0;                    // by this wave, in this call
     wave_priority( wp, 0 );
     while( ( act[ 0 ] | act[ 1 ] ) != 0u )
@@ -576,8 +576,9 @@ ST_DEV void track_lean_lane2( const PA& pa, const int32_t* list, int64_t n_lanes
                         auto const& P = pa.view();
                         store_hot_state( P, j, L[ k ], __longlong_as_double( ( long long )s_rec ), 0,
                                          P.at_element[ j ] + ri( r, 1 ), P.at_turn[ j ] + turns_done );
-                        if( fail ) status = -1;
                     }
+                    // fail depends on the record alone: the same in every lane
+                    failed = failed || fail;
                     act[ k ] &= ~lost;
                 }
             }
@@ -603,7 +604,7 @@ ST_DEV void track_lean_lane2( const PA& pa, const int32_t* list, int64_t n_lanes
         if( ( act[ 0 ] | act[ 1 ] ) != 0u ) wave_priority( wp, turns_done );
     }
     wave_priority_leave( wp, turns_done );
-    if( status != 0 ) atomicOr( status_out, status );
+    if( failed && __lane_id() == 0 ) atomicOr( status_out, -1 );
 }
 
 // Indices of the particles a track_until call has work for (state 1 and
