@@ -34,9 +34,9 @@
  * list is rebuilt between them; results identical), "queues=1|8" (default
  * 1: one work queue for the persistent launch; 8: one per XCD),
  * "ppl=1|2|auto" (default auto: particles per lane of the until-turn kernel
- * of lean lattices; auto takes two for a request whose particles fill at
- * least 70 % of the resident waves with 128-particle tiles and one
- * otherwise; results identical), "balance=0|1" (default 1: with compact=1
+ * of lean lattices; auto takes two in the FMA build for a request whose
+ * particles fill at least 70 % of the resident waves with 128-particle
+ * tiles and one otherwise; results identical), "balance=0|1" (default 1: with compact=1
  * every launch gives each resident wave one 64-particle tile, the
  * particles furthest behind first, and the host reads the counts back
  * between launches; 0: every chunk hands out all its tiles; results

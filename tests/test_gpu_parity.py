@@ -266,8 +266,9 @@ def test_two_particles_per_lane_after_track_line_match_oracle():
 
 @pytest.mark.parametrize("fma", [False, True])
 def test_ppl_auto_matches_one_particle_per_lane(fma):
-    """ppl=auto (the default) takes two particles per lane for a request
-    whose particles fill at least 70 % of the resident waves with
+    """ppl=auto (the default) takes two particles per lane in the FMA build
+    (the exact build keeps one) for a request whose particles fill at least
+    70 % of the resident waves with
     128-particle tiles (22,938 with waves_per_cu=1, 256 resident waves).
     40,000 particles to 200 sigma start above it and fall below it in the
     first request (8,403 left after 3 turns); 35,840 to 20 sigma stay above
