@@ -372,6 +372,35 @@ ST_DEV bool lean_other_hot( unsigned op, cptr_t e, Lane& L, const PA& pa, int64_
     return false;
 }
 
+// The same for the pair: one dispatch, each map applied to both particles
+// (ok0 / ok1 their survival predicates).
+template< class PA >
+ST_DEV void lean_other_hot2( unsigned op, cptr_t e, Lane& L0, Lane& L1, const PA& pa, int64_t i0,
+                             int64_t i1, bool active0, bool active1, bool& fail, bool& ok0, bool& ok1 )
+{
+    ok0 = ok1 = true;
+    switch( op )
+    {
+    case ST_OP_XYSHIFT: xyshift( L0, e ); xyshift( L1, e ); return;
+    case ST_OP_SROTATION: srotation( L0, e ); srotation( L1, e ); return;
+    case ST_OP_LIMIT_RECT: ok0 = limit_rect_ok( L0, e ); ok1 = limit_rect_ok( L1, e ); return;
+    case ST_OP_LIMIT_ELLIPSE: ok0 = limit_ellipse_ok( L0, e ); ok1 = limit_ellipse_ok( L1, e ); return;
+    case ST_OP_LIMIT_RECT_ELLIPSE:
+        ok0 = limit_rect_ellipse_ok( L0, e );
+        ok1 = limit_rect_ellipse_ok( L1, e );
+        return;
+    case ST_OP_DIPEDGE: dipedge( L0, e ); dipedge( L1, e ); return;
+    case ST_OP_DRIFT_EXACT:
+    case ST_OP_CAVITY:
+        ok0 = lean_other_hot( op, e, L0, pa, i0, active0, fail );
+        ok1 = lean_other_hot( op, e, L1, pa, i1, active1, fail );
+        return;
+    default: break;
+    }
+    fail = true;
+    ok0 = ok1 = false;
+}
+
 // Horner recurrence of two particles through one record: the coefficient
 // reads and the order dispatch are shared, the two chains interleave.
 template< int N >
@@ -561,13 +590,13 @@ ST_DEV void track_lean_lane2( const PA& pa, const int32_t* list, int64_t n_lanes
             uint32_t const noff = off + ST_REC_BYTES( h );
             uint64_t const hn = rec_at( lat, noff )[ 0 ];
             if( op == ST_OP_DRIFT_EXACT ) s_rec = r[ 3 ];
+            bool fail = false, ok[ K ];
+            lean_other_hot2( op, r + ST_REC_PAYLOAD, L[ 0 ], L[ 1 ], pa, idx[ 0 ], idx[ 1 ],
+                             lane_in( act[ 0 ] ), lane_in( act[ 1 ] ), fail, ok[ 0 ], ok[ 1 ] );
 #pragma unroll
             for( int k = 0; k < K; ++k )
             {
-                bool fail = false;
-                bool const ok = lean_other_hot( op, r + ST_REC_PAYLOAD, L[ k ], pa, idx[ k ],
-                                                lane_in( act[ k ] ), fail );
-                uint64_t const lost = ballot( !ok ) & act[ k ];
+                uint64_t const lost = ballot( !ok[ k ] ) & act[ k ];
                 if( lost != 0u )
                 {
                     if( lane_in( lost ) )
