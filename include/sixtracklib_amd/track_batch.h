@@ -5,8 +5,10 @@
  * 360-376) and the CUDA job, which asserts a single particle set
  * (cuda/track/track_job.cpp:2347); a user of the reference tracks N seeds
  * with N jobs one after the other.  Here N (lattice, particle-set) pairs
- * are tracked by one persistent launch per chunk of turns, the chunks of a
- * track_until call captured in one HIP graph.  Each seed's result is
+ * are tracked by the same persistent launches: by default every launch
+ * gives each resident wave one tile of any seed, the particles furthest
+ * behind first (balance=1); with balance=0 the chunks of turns of a
+ * track_until call are captured in one HIP graph.  Each seed's result is
  * bit-identical to a plain NS(TrackJob_track_until) on that seed (same
  * per-particle code; fma=1 selects the fused build for both).
  *
@@ -31,12 +33,12 @@
  *                         (scalar) or copied piece by piece into LDS by
  *                         256-lane workgroups that walk it together (lds;
  *                         one particle per lane; results identical)
- *   balance=0|1           0: chunks of turns, every wave taking tiles until
- *                         none are left (graph-captured); 1: launches of one
- *                         tile per resident wave, the particles furthest
- *                         behind first, the host reading the counts back
- *                         between launch groups (no graph; results
- *                         identical)
+ *   balance=0|1           1 (default): launches of one tile per resident
+ *                         wave, the particles furthest behind first, the
+ *                         host reading the counts back between launch
+ *                         groups (no graph); 0: chunks of turns, every wave
+ *                         taking tiles until none are left (graph-captured);
+ *                         results identical
  * NS(TrackBatch_autotune) picks block, waves_per_cu, queues, particles per
  * lane and lattice staging by timing short trials on the batch itself, the
  * three fastest twice (state restored afterwards).

@@ -351,6 +351,7 @@ NS(TrackBatch)* NS(TrackBatch_new)( int64_t n_seeds, NS(Buffer)* const* lattices
         return nullptr;
     }
     Config cfg;
+    cfg.balance = true;                  // default: launches of one tile per resident wave
     if( !cfg.parse( config_str ) ) return nullptr;
     NS(TrackBatch)* b = new NS(TrackBatch)();
     b->lat_bufs.assign( lattices, lattices + n_seeds );

@@ -5,8 +5,9 @@ The reference has no batched form; its users run one job per seed.  The
 parity bar is therefore: every seed of a batch ends bit-identical to a
 plain TrackJob on that seed alone (same build), and, for the exact build,
 to the CPU oracle -- with uneven particle counts (not multiples of 64, one
-seed smaller than a wave), chunked launches, XCD-grouped and single
-queues, with and without graph capture, and after the autotuner.
+seed smaller than a wave), balanced launches (the default) and
+graph-captured chunks, XCD-grouped and single queues, one and two
+particles per lane, and after the autotuner.
 """
 import numpy as np
 import pytest
@@ -61,11 +62,13 @@ def _plain(rings, parts, turns, fma):
     return out
 
 
-@pytest.mark.parametrize("cfg", ["fma=0", "fma=0 queues=1 graph=0", "fma=0 turns_per_launch=2",
-                                 "fma=0 block=64 waves_per_cu=16 turns_per_launch=3",
-                                 "fma=0 ppl=2 waves_per_cu=20 turns_per_launch=2",
-                                 "fma=0 balance=1", "fma=0 balance=1 waves_per_cu=1 queues=8",
-                                 "fma=0 staging=lds", "fma=0 staging=lds turns_per_launch=2 queues=1",
+@pytest.mark.parametrize("cfg", ["fma=0", "fma=0 balance=0", "fma=0 balance=0 queues=1 graph=0",
+                                 "fma=0 balance=0 turns_per_launch=2",
+                                 "fma=0 balance=0 block=64 waves_per_cu=16 turns_per_launch=3",
+                                 "fma=0 balance=0 ppl=2 turns_per_launch=2", "fma=0 ppl=2",
+                                 "fma=0 balance=1 waves_per_cu=1 queues=8",
+                                 "fma=0 balance=0 staging=lds",
+                                 "fma=0 balance=0 staging=lds turns_per_launch=2 queues=1",
                                  "fma=0 staging=lds balance=1 waves_per_cu=4"])
 def test_batch_matches_oracle_exact(cfg):
     from sixtracklib_amd.trackbatch import TrackBatch
