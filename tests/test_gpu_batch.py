@@ -91,10 +91,12 @@ def test_batch_matches_oracle_exact(cfg):
     assert 0 < lost < sum(COUNTS)
 
 
-@pytest.mark.parametrize("cfg", ["fma=1 turns_per_launch=2", "fma=1 staging=lds turns_per_launch=2"])
+@pytest.mark.parametrize("cfg", ["fma=1 turns_per_launch=2", "fma=1 staging=lds turns_per_launch=2",
+                                 "fma=1 ppl=2", "fma=1 ppl=2 balance=0 turns_per_launch=3"])
 def test_batch_fma_matches_plain_fma_bitwise(cfg):
-    """FMA build: the batch (scalar-load or LDS-staged lattice) equals the
-    plain FMA job bit for bit."""
+    """FMA build: the batch (scalar-load or LDS-staged lattice, one or two
+    particles per lane, balanced or chunked launches) equals the plain FMA
+    job bit for bit."""
     from sixtracklib_amd.trackbatch import TrackBatch
     rings, parts = _workload()
     plain_in = [_copy(pb) for pb in parts]
