@@ -36,7 +36,10 @@
  * "ppl=1|2|auto" (default auto: particles per lane of the until-turn kernel
  * of lean lattices; auto takes two in the FMA build for a request whose
  * particles fill at least 70 % of the resident waves with 128-particle
- * tiles and one otherwise; results identical), "balance=0|1" (default 1: with compact=1
+ * tiles and one otherwise, and between one and two particles per resident
+ * lane it runs mixed launches, some waves of every SIMD with two particles
+ * per lane and the others with one; results identical), "mix=0|1" (default
+ * 1: the mixed launches of ppl=auto), "balance=0|1" (default 1: with compact=1
  * every launch gives each resident wave one 64-particle tile, the
  * particles furthest behind first, and the host reads the counts back
  * between launches; 0: every chunk hands out all its tiles; results

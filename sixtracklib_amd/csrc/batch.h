@@ -49,7 +49,10 @@ st_batch_sched_t;
         int64_t max_particles, int64_t until_turn, hipStream_t stream );              \
     extern "C" hipError_t st_launch_batch_track##SUF( const st_batch_sched_t* s,     \
         int64_t until_turn, int64_t max_turns, int32_t* status, int block, int grid, \
-        int particles_per_lane, int heavy, int staging, hipStream_t stream );
+        int particles_per_lane, int heavy, int staging, hipStream_t stream );          \
+    extern "C" hipError_t st_launch_batch_track_mix##SUF( const st_batch_sched_t* s, \
+        int64_t until_turn, int64_t max_turns, int32_t* status, int grid,            \
+        int64_t pair_waves, hipStream_t stream );
 
 ST_DECLARE_BATCH_LAUNCHERS( _exact )
 ST_DECLARE_BATCH_LAUNCHERS( _fma )

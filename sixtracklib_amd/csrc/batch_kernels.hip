@@ -361,6 +361,54 @@ KN( st_k_batch_track2 )( st_batch_sched_t s, int64_t until_turn, int64_t max_tur
     }
 }
 
+// Mixed launch of one seed whose active particles all sit at one turn (the
+// front part of its list, split[0] of them): waves 0 .. pair_waves - 1 take
+// 128 particles each, two per lane (track_lean_lane2), the others share the
+// rest evenly, one per lane (track_lean_lane).  With a grid of 8 workgroups
+// of 256 per CU the dispatcher gives every SIMD one wave of each run of
+// n_cu consecutive workgroups, so pair_waves = m * 4 * n_cu puts m pair
+// waves and 8 - m single waves on every SIMD: all resident waves busy, the
+// same work on every SIMD, where one particle per lane would need a second,
+// part-filled launch and two per lane would leave waves idle.  The issue
+// priority keeps the waves level in turns (the pairs, slower per turn, go
+// first when behind).
+__global__ void __launch_bounds__( 256, ST_LEAN_WAVES_PER_EU )
+KN( st_k_batch_track_mix )( st_batch_sched_t s, int64_t until_turn, int64_t max_turns,
+                            int32_t* status, int64_t pair_waves )
+{
+    ci32_t* const split = ( ci32_t* )s.split;
+    int64_t const n = split[ 0 ];
+    int64_t const W = ( int64_t )gridDim.x * ( blockDim.x >> 6 );
+    int64_t const w = ( int64_t )blockIdx.x * ( blockDim.x >> 6 ) + ( threadIdx.x >> 6 );
+    cptr_t const sd = ( cptr_t )s.seeds;                      // seed 0
+    const int32_t* const list = ( const int32_t* )sd[ 24 ];
+    cptr_t const lat = ( cptr_t )sd[ 22 ];
+    int64_t const n_elem = ( int64_t )sd[ 23 ];
+    WavePrio wp;
+    if( s.wave_prio && max_turns > 0 && s.progress != nullptr )
+    {
+        wp.k = max_turns;
+        wp.progress = s.progress;
+        wp.n_waves = W;
+    }
+    int64_t const pairs = pair_waves < W ? pair_waves : W;
+    if( w < pairs )
+    {
+        int64_t const begin = 128 * w;
+        int64_t const end = begin + 128 < n ? begin + 128 : n;
+        track_lean_lane2( PDesc{ sd }, list, end, begin, lat, n_elem, until_turn, max_turns, status, wp );
+        return;
+    }
+    int64_t const U = W - pairs;                               // single waves
+    int64_t const rest = n > 128 * pairs ? n - 128 * pairs : 0;
+    int64_t const u = w - pairs;
+    int64_t const q = rest / U, r = rest % U;
+    int64_t const begin = 128 * pairs + u * q + ( u < r ? u : r );
+    int64_t const end = begin + q + ( u < r ? 1 : 0 );
+    track_lean_lane< false >( PDesc{ sd }, list, end, begin + __lane_id(), lat, n_elem, until_turn,
+                              max_turns, status, wp );
+}
+
 // ------------------------------------------------------------ launchers
 
 extern "C" hipError_t KN( st_launch_batch_reset )( const st_batch_sched_t* s, hipStream_t stream )
@@ -391,6 +439,16 @@ extern "C" hipError_t KN( st_launch_batch_select_split )( const st_batch_sched_t
     if( e != hipSuccess ) return e;
     hipLaunchKernelGGL( KN( st_k_batch_select_split ), dim3( gx, ( unsigned )s->n_seeds ),
                         dim3( 256 ), 0, stream, *s, until_turn );
+    return hipGetLastError();
+}
+
+extern "C" hipError_t KN( st_launch_batch_track_mix )( const st_batch_sched_t* s,
+    int64_t until_turn, int64_t max_turns, int32_t* status, int grid, int64_t pair_waves,
+    hipStream_t stream )
+{
+    if( s->n_seeds != 1 || s->split == nullptr || grid <= 0 ) return hipErrorInvalidValue;
+    hipLaunchKernelGGL( KN( st_k_batch_track_mix ), dim3( ( unsigned )grid ), dim3( 256 ), 0, stream,
+                        *s, until_turn, max_turns, status, pair_waves );
     return hipGetLastError();
 }
 

@@ -63,6 +63,8 @@ struct Config
     bool ppl_auto = false;       // balanced form: per request, 2 when the particles
                                  // fill at least 70 % of the resident waves with
                                  // 128-particle tiles, else 1 (see run_balanced)
+    bool mix = true;             // with ppl_auto: mixed launches of one and two particles
+                                 // per lane between 64 and 70 % x 128 per resident wave
     bool heavy = false;          // lattices hold maps outside the lean set
                                  // (set by the owner, one particle per lane)
     bool balance = false;        // run_balanced() available (split lists)
@@ -114,6 +116,7 @@ struct Config
             ppl = ( int )iv;
             ppl_auto = false;
         }
+        else if( k == "mix" ) mix = iv != 0;
         else if( k == "staging" )
         {
             if( v == "scalar" || v == "0" ) staging = 0;
@@ -488,17 +491,20 @@ struct BatchRunner
     // 1M particles and 100-turn requests, 34 turns per launch are 3 %
     // faster than 23; 50 are faster still where few particles are lost and
     // slower where more are, their lanes idling to the end of the launch).
+    int64_t equal_turns( int64_t r, int64_t n_active ) const
+    {
+        r = std::max< int64_t >( r, 1 );
+        int64_t const kmax = std::min( std::max< int64_t >( 3 * chunk_turns( n_active ) / 2, 1 ), r );
+        int64_t const v = ( r + kmax - 1 ) / kmax;
+        return ( r + v - 1 ) / v;
+    }
+
     int64_t balanced_turns( int64_t tiles, int64_t r, int64_t n_active ) const
     {
         if( cfg.turns_per_launch > 0 ) return cfg.turns_per_launch;
         r = std::max< int64_t >( r, 1 );
         int64_t const W = std::max< int64_t >( waves(), 1 );
-        if( tiles <= W )
-        {
-            int64_t const kmax = std::min( std::max< int64_t >( 3 * chunk_turns( n_active ) / 2, 1 ), r );
-            int64_t const v = ( r + kmax - 1 ) / kmax;
-            return ( r + v - 1 ) / v;
-        }
+        if( tiles <= W ) return equal_turns( r, n_active );
         int64_t const kmax = std::min( chunk_turns( W * cfg.tile_particles() ), r );
         int64_t const vmin = ( r + kmax - 1 ) / kmax;
         int64_t best_k = kmax, best_cost = INT64_MAX;
@@ -558,6 +564,39 @@ struct BatchRunner
         return t;
     }
 
+    // Mixed launches (st_k_batch_track_mix): for one seed whose n active
+    // particles sit at one turn, more than one per lane of every resident
+    // wave and fewer than two, the pair waves that make every wave of the
+    // full grid busy: m per SIMD (m * 4 * n_cu in all), m the fewest that
+    // hold n.  0 when the launch does not apply (several seeds, particles
+    // at several turns, a grid other than 8 workgroups of 256 per CU).
+    int64_t mix_pairs( int64_t n ) const
+    {
+        if( !cfg.mix || !cfg.ppl_auto || cfg.heavy || cfg.lds() || n_seeds() != 1 ||
+            h_split == nullptr || h_split[ 1 ] != 0 || cfg.block != 256 || cfg.waves_per_cu != 32 ||
+            n_cu <= 0 ) return 0;
+        int64_t const W = waves();
+        if( W != 32 * ( int64_t )n_cu || n <= 64 * W ) return 0;
+        int64_t const per = 64 * 4 * ( int64_t )n_cu;          // particles one pair wave per SIMD adds
+        int64_t const m = ( n - 64 * W + per - 1 ) / per;
+        if( m >= 8 ) return 0;
+        // the particles must fill at least 95 % of the lanes the mixed launch
+        // holds: the single waves share what the pairs leave, and part-filled
+        // ones cost as much as full ones
+        int64_t const cap = 64 * W + m * per;
+        return 100 * n >= 95 * cap ? m * 4 * ( int64_t )n_cu : 0;
+    }
+
+    bool enqueue_track_mix( int64_t until_turn, int64_t k, int32_t* d_status, int64_t pairs,
+                            hipStream_t stream ) const
+    {
+        st_batch_sched_t const s = sched_split();
+        hipError_t const e = cfg.fma
+            ? st_launch_batch_track_mix_fma( &s, until_turn, k, d_status, grid(), pairs, stream )
+            : st_launch_batch_track_mix_exact( &s, until_turn, k, d_status, grid(), pairs, stream );
+        return hip_ok( e, "launch batch track (mixed)" );
+    }
+
     // Balanced form, one select pass read back (synchronises the stream):
     // particles with turns to go, the smallest at_turn, and the tiles of
     // the split lists.
@@ -595,26 +634,42 @@ struct BatchRunner
     {
         if( !staging_ok() ) return false;
         int64_t n_active = 0, min_turn = 0, tiles = 0, k = 0, limit = 0, launches = 0;
+        bool decide = true;                    // choose the launch form for this group
         int64_t const W = std::max< int64_t >( waves(), 1 );
         for( int64_t group = 0;; ++group )
         {
             if( !select_split_sync( until_turn, stream, n_active, min_turn, tiles ) ) return false;
             if( n_active == 0 ) return true;
-            if( group == 0 )
+            int64_t pairs = 0;                 // mixed launch: pair waves (0: none)
+            if( decide )
             {
                 // ppl=auto: two particles per lane run the record walk once for
                 // both (8 % more particle-turns per second on the 20,000-element
                 // ring with 75 % or more of the resident waves busy), but hold
                 // half the waves of one per lane (at 65 % they are 4 % slower);
                 // they are chosen when the particles fill at least 70 % of the
-                // resident waves with 128-particle tiles
+                // resident waves with 128-particle tiles.  Between one and two
+                // particles per resident lane a mixed launch (mix_pairs) keeps
+                // every wave busy instead; it is decided afresh before each
+                // mixed launch, as particles are lost.
                 if( cfg.ppl_auto && !cfg.heavy && !cfg.lds() )
                 {
                     cfg.ppl = ( 100 * n_active >= 70 * 128 * W ) ? 2 : 1;
                     tiles = split_tiles();
+                    pairs = mix_pairs( n_active );
                 }
-                k = balanced_turns( tiles, until_turn - min_turn, n_active );
-                limit = 16 + 4 * ( ( tiles + W - 1 ) / W ) * ( ( until_turn - min_turn + k - 1 ) / k );
+                int64_t const r = until_turn - min_turn;
+                k = pairs > 0 ? equal_turns( r, n_active ) : balanced_turns( tiles, r, n_active );
+                if( group == 0 ) limit = 16 + 4 * ( ( tiles + W - 1 ) / W ) * ( ( r + k - 1 ) / k );
+                decide = pairs > 0;
+            }
+            if( pairs > 0 )
+            {
+                // one mixed launch per group: every active particle advances k
+                // turns, and the next group decides again
+                if( launches++ >= limit ) return fail( "balanced tracking made no progress" );
+                if( !enqueue_track_mix( until_turn, k, d_status, pairs, stream ) ) return false;
+                continue;
             }
             int64_t const v = ( until_turn - min_turn + k - 1 ) / k;     // visits per particle
             int64_t const n = std::max( v, ( tiles * v + W - 1 ) / W );

@@ -225,6 +225,7 @@ struct NS(TrackJobBase)
     // XCD the waves of the first XCD would take every tile of a small set
     int queues = 1;
     int ppl = 0;                         // particles per lane of the lean until path (0: auto)
+    bool mix = true;                     // with ppl auto: mixed one/two-particle launches
     bool balance = true;                 // one tile per wave and launch, furthest behind first
     bool wave_prio = true;               // waves with more turns left issue first
     int staging = 0;                     // lean until path: lattice via scalar loads or LDS
@@ -645,6 +646,7 @@ struct NS(TrackJobBase)
         // cover: the same rate with every wave busy, 5 % slower over the
         // 1000-turn job of the 20,000-element ring)
         c.ppl_auto = !lat_full && ppl == 0 && staging == 0 && fma;
+        c.mix = mix;
         c.balance = balance;
         c.wave_prio = wave_prio;
         c.staging = ( lat_full || c.ppl == 2 ) ? 0 : staging;
@@ -847,6 +849,7 @@ struct NS(TrackJobBase)
             else if( k == "turns_per_launch" ) turns_per_launch = iv > 0 ? iv : 0;
             else if( k == "queues" ) queues = ( iv == 8 ) ? 8 : 1;
             else if( k == "ppl" ) ppl = ( v == "auto" ) ? 0 : ( iv == 2 ) ? 2 : 1;
+            else if( k == "mix" ) mix = iv != 0;
             else if( k == "balance" ) balance = iv != 0;
             else if( k == "wave_prio" ) wave_prio = iv != 0;
             else if( k == "staging" ) staging = ( v == "lds" || v == "1" ) ? 1 : 0;

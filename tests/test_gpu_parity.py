@@ -293,6 +293,43 @@ def test_ppl_auto_matches_one_particle_per_lane(fma):
         assert_particles_equal(res["auto"], res["one"], what="auto vs ppl=1, %d particles" % (64 * n_amp))
 
 
+@pytest.mark.parametrize("max_sigma,later", [(20.0, False), (200.0, False), (20.0, True)])
+def test_mixed_launches_match_one_particle_per_lane(max_sigma, later):
+    """With ppl=auto in the FMA build, a request of one particle set whose
+    active particles sit at one turn and number between 64 and 70 % x 128
+    per resident wave (524,288 to 734,003 with 8,192 waves) runs mixed
+    launches: some waves of every SIMD take two particles per lane, the
+    others share the rest one per lane.  600,000 particles on a 400-element
+    ring: to 20 sigma (all requests mixed), to 200 sigma (most lost in the
+    first turns, the later requests one per lane), and with every third
+    particle starting two turns later (the first request not mixed, the
+    next ones mixed).  Results equal the one-particle kernel's bit for
+    bit."""
+    from sixtracklib_amd.trackjob import TrackJob
+    from sixtracklib_amd.particles import particles_in
+    eb = lattices.lhc_like_ring(seed=3, n_cells=8, tune=8.31 * 8 / 40)
+    sig = lattices.twiss_sigma(beta=236.0)
+    pb, _ = lattices.polar_grid(9375, 64, sigma_x=sig, sigma_y=sig, max_sigma=max_sigma,
+                                delta=2.7e-4)
+    if later:
+        p = particles_in(pb)
+        at = np.array(p.at_turn)
+        at[::3] = 2
+        p.at_turn[:] = at
+    res = {}
+    for key, cfg in (("mixed", "fma=1"), ("one", "fma=1 ppl=1")):
+        buf = _copy(pb)
+        job = TrackJob(eb, buf, config_str=cfg)
+        for t in (3, 7, 12):
+            assert job.track_until(t).last_track_status == 0
+        job.collect()
+        res[key] = _pdict(buf)
+    lost = np.count_nonzero(res["one"]["state"] == 0)
+    if max_sigma > 100:
+        assert lost > 100000
+    assert_particles_equal(res["mixed"], res["one"], what="mixed vs ppl=1")
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_kitchen_sink_exact_bitwise(seed):
     eb = lattices.kitchen_sink(seed)
