@@ -580,11 +580,11 @@ struct BatchRunner
         int64_t const per = 64 * 4 * ( int64_t )n_cu;          // particles one pair wave per SIMD adds
         int64_t const m = ( n - 64 * W + per - 1 ) / per;
         if( m >= 8 ) return 0;
-        // the particles must fill at least 95 % of the lanes the mixed launch
+        // the particles must fill at least 90 % of the lanes the mixed launch
         // holds: the single waves share what the pairs leave, and part-filled
         // ones cost as much as full ones
         int64_t const cap = 64 * W + m * per;
-        return 100 * n >= 95 * cap ? m * 4 * ( int64_t )n_cu : 0;
+        return 100 * n >= 90 * cap ? m * 4 * ( int64_t )n_cu : 0;
     }
 
     bool enqueue_track_mix( int64_t until_turn, int64_t k, int32_t* d_status, int64_t pairs,
