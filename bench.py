@@ -45,11 +45,11 @@ HBM_PEAK_GBS = 8000.0
 # (KiB per dispatch; a 1 GiB device-to-device copy reads 524,308 KiB of
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
 # a track launch of the plain 1000-turn job of the 1M-particle ring (mean of
-# its 37 launches, 15 of two particles per lane and 22 of one: 3,035,851 /
-# 2,808,437 KiB) and one turn of element-by-element
+# its 25 launches, 3 of two particles per lane, 18 mixed and 4 of one:
+# 4,312,180 / 4,478,483 KiB) and one turn of element-by-element
 # tracking of the collective ring with 100k particles in the exact build
 # (one 10-turn call / 10: 6,505,271 / 2,051,917,580 KiB)
-TRACK_LAUNCH_HBM_BYTES = 3035851 * 2048 + 2808437 * 1024
+TRACK_LAUNCH_HBM_BYTES = 4312180 * 2048 + 4478483 * 1024
 EBE_TURN_HBM_BYTES = (6505271 * 2048 + 2051917580 * 1024) // 10
 
 
@@ -508,13 +508,14 @@ def main():
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "traffic": TRACK_LAUNCH_HBM_BYTES,
-                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of st_k_batch_track2_fma "
-                                       "and st_k_batch_track_fma on this workload, mean of the 37 "
+                     "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of the track launches "
+                                       "(st_k_batch_track2_fma, st_k_batch_track_mix_fma, "
+                                       "st_k_batch_track_fma) on this workload, mean of the 25 "
                                        "launches of the 1000-turn job, scaled by a 1 GiB device copy "
-                                       "(FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x); the two-particle "
-                                       "launches read the particle constants and delta and write "
-                                       "delta and psigma at each Cavity (registers hold only the "
-                                       "Drift/Multipole state): 20.3 GB per launch at 64 GB/s",
+                                       "(FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x); waves with two "
+                                       "particles per lane read the particle constants and delta and "
+                                       "write delta and psigma at each Cavity (registers hold only the "
+                                       "Drift/Multipole state): 14-22 GB per launch at 47-65 GB/s",
                      "algorithmic_hbm_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "batched": batched,
