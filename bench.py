@@ -48,9 +48,9 @@ HBM_PEAK_GBS = 8000.0
 # its 25 launches, 3 of two particles per lane, 18 mixed and 4 of one:
 # 4,312,180 / 4,478,483 KiB) and one turn of element-by-element
 # tracking of the collective ring with 100k particles in the exact build
-# (one 10-turn call / 10: 6,505,271 / 2,051,917,580 KiB)
+# (one 10-turn call / 10: 1,473,305 / 2,052,356,320 KiB)
 TRACK_LAUNCH_HBM_BYTES = 4312180 * 2048 + 4478483 * 1024
-EBE_TURN_HBM_BYTES = (6505271 * 2048 + 2051917580 * 1024) // 10
+EBE_TURN_HBM_BYTES = (1473305 * 2048 + 2052356320 * 1024) // 10
 
 
 def sigma_at_start(eb):

@@ -72,6 +72,9 @@ ST_DEV void store_lane( const st_dev_particles_t& P, int64_t i, const Lane& L,
 // Without ``constants`` the eight fields that are fixed for a particle
 // (q0, mass0, beta0, gamma0, p0c, chi, charge_ratio, particle_id) are not
 // written: the caller knows the slot already holds this particle's values.
+// The stores are non-temporal: the kernel never reads a record back, and
+// streaming stores keep their rate with the element maps issuing between
+// them (plain stores lose about 8 % of the store rate there).
 ST_DEV void store_record_narrow( const st_dev_particles_t& O, int64_t k, double gamma0,
     const Lane& L, const LaneConst& C, int64_t pid, int64_t at_element, int64_t at_turn,
     bool constants )
@@ -81,22 +84,28 @@ ST_DEV void store_record_narrow( const st_dev_particles_t& O, int64_t k, double 
     // rather than the loop carrying one running 64-bit address per field
     uint32_t b = static_cast< uint32_t >( k ) * 8u;
     asm volatile( "" : "+v"( b ) );
-    auto d = [ b ]( double* base ) -> double&
-    { return *reinterpret_cast< double* >( reinterpret_cast< char* >( base ) + b ); };
-    auto n = [ b ]( int64_t* base ) -> int64_t&
-    { return *reinterpret_cast< int64_t* >( reinterpret_cast< char* >( base ) + b ); };
+    auto d = [ b ]( double* base, double v )
+    {
+        __builtin_nontemporal_store(
+            v, reinterpret_cast< double* >( reinterpret_cast< char* >( base ) + b ) );
+    };
+    auto n = [ b ]( int64_t* base, int64_t v )
+    {
+        __builtin_nontemporal_store(
+            v, reinterpret_cast< int64_t* >( reinterpret_cast< char* >( base ) + b ) );
+    };
     if( constants )
     {
-        d( O.q0 ) = C.q0; d( O.mass0 ) = C.mass0; d( O.beta0 ) = C.beta0;
-        d( O.gamma0 ) = gamma0; d( O.p0c ) = C.p0c; d( O.chi ) = C.chi;
-        d( O.charge_ratio ) = C.charge_ratio; n( O.particle_id ) = pid;
+        d( O.q0, C.q0 ); d( O.mass0, C.mass0 ); d( O.beta0, C.beta0 );
+        d( O.gamma0, gamma0 ); d( O.p0c, C.p0c ); d( O.chi, C.chi );
+        d( O.charge_ratio, C.charge_ratio ); n( O.particle_id, pid );
     }
-    d( O.s ) = L.s;
-    d( O.x ) = L.x; d( O.y ) = L.y; d( O.px ) = L.px; d( O.py ) = L.py;
-    d( O.zeta ) = L.zeta; d( O.psigma ) = L.psigma; d( O.delta ) = L.delta;
-    d( O.rpp ) = L.rpp; d( O.rvv ) = L.rvv;
-    n( O.at_element ) = at_element; n( O.at_turn ) = at_turn;
-    n( O.state ) = L.state;
+    d( O.s, L.s );
+    d( O.x, L.x ); d( O.y, L.y ); d( O.px, L.px ); d( O.py, L.py );
+    d( O.zeta, L.zeta ); d( O.psigma, L.psigma ); d( O.delta, L.delta );
+    d( O.rpp, L.rpp ); d( O.rvv, L.rvv );
+    n( O.at_element, at_element ); n( O.at_turn, at_turn );
+    n( O.state, L.state );
 }
 
 // Elements other than Drift and Multipole (track/track.h:209-585).
