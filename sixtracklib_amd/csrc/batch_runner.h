@@ -564,36 +564,62 @@ struct BatchRunner
         return t;
     }
 
-    // Mixed launches (st_k_batch_track_mix): for one seed whose n active
-    // particles sit at one turn, more than one per lane of every resident
-    // wave and fewer than two, the pair waves that make every wave of the
-    // full grid busy: m per SIMD (m * 4 * n_cu in all), m the fewest that
-    // hold n.  0 when the launch does not apply (several seeds, particles
-    // at several turns, a grid other than 8 workgroups of 256 per CU).
-    int64_t mix_pairs( int64_t n ) const
+    // Mixed launches (st_k_batch_track_mix) for one seed whose n active
+    // particles sit at one turn: ``pairs`` waves of the grid take 128
+    // particles each, two per lane, and the other waves share the rest
+    // evenly, one per lane.  The dispatcher places one wave of each run of
+    // 256 consecutive workgroups of 256 on every SIMD, so a grid of w * n_cu
+    // workgroups holds w waves on every SIMD, the pair waves m of them.
+    //   * More than one particle per resident lane (n > 64 W, W = 32 waves
+    //     per CU): the full grid (w = 8), m the fewest pair waves per SIMD
+    //     that hold n, when n fills at least 90 % of the lanes (part-filled
+    //     single waves cost as much as full ones).
+    //   * At most one per resident lane: j = ceil(n / (64 x 4 x n_cu))
+    //     waves per SIMD hold n one per lane, and one wave per 64-particle
+    //     tile would leave some SIMDs with j waves and the others with
+    //     j - 1, idle once their waves end.  For j = 6 or 7 the launch
+    //     spreads n evenly over j waves on every SIMD (m = 0); for j = 8,
+    //     over 6 waves of which m = 2 pairs (the same lanes, 4 % to 5 %
+    //     faster than one per lane when the tiles leave 9 % of the lanes
+    //     empty, equal when they fill them).
+    // false when the launch does not apply (several seeds, particles at
+    // several turns, a grid other than 8 workgroups of 256 per CU, fewer
+    // than 6 waves per SIMD).
+    bool mix_shape( int64_t n, int64_t& pairs, int& grid_wgs ) const
     {
+        pairs = 0;
+        grid_wgs = 0;
         if( !cfg.mix || !cfg.ppl_auto || cfg.heavy || cfg.lds() || n_seeds() != 1 ||
             h_split == nullptr || h_split[ 1 ] != 0 || cfg.block != 256 || cfg.waves_per_cu != 32 ||
-            n_cu <= 0 ) return 0;
+            n_cu <= 0 || n <= 0 ) return false;
         int64_t const W = waves();
-        if( W != 32 * ( int64_t )n_cu || n <= 64 * W ) return 0;
-        int64_t const per = 64 * 4 * ( int64_t )n_cu;          // particles one pair wave per SIMD adds
-        int64_t const m = ( n - 64 * W + per - 1 ) / per;
-        if( m >= 8 ) return 0;
-        // the particles must fill at least 90 % of the lanes the mixed launch
-        // holds: the single waves share what the pairs leave, and part-filled
-        // ones cost as much as full ones
-        int64_t const cap = 64 * W + m * per;
-        return 100 * n >= 90 * cap ? m * 4 * ( int64_t )n_cu : 0;
+        if( W != 32 * ( int64_t )n_cu ) return false;
+        int64_t const per = 64 * 4 * ( int64_t )n_cu;          // one wave per SIMD, one per lane
+        if( n > 64 * W )
+        {
+            int64_t const m = ( n - 64 * W + per - 1 ) / per;
+            if( m >= 8 ) return false;
+            int64_t const cap = 64 * W + m * per;
+            if( 100 * n < 90 * cap ) return false;
+            pairs = m * 4 * ( int64_t )n_cu;
+            grid_wgs = grid();
+            return true;
+        }
+        int64_t const j = ( n + per - 1 ) / per;
+        if( j < 6 ) return false;
+        int64_t const m = j == 8 ? 2 : 0;
+        pairs = m * 4 * ( int64_t )n_cu;
+        grid_wgs = ( int )( ( j - m ) * n_cu );
+        return true;
     }
 
     bool enqueue_track_mix( int64_t until_turn, int64_t k, int32_t* d_status, int64_t pairs,
-                            hipStream_t stream ) const
+                            int grid_wgs, hipStream_t stream ) const
     {
         st_batch_sched_t const s = sched_split();
         hipError_t const e = cfg.fma
-            ? st_launch_batch_track_mix_fma( &s, until_turn, k, d_status, grid(), pairs, stream )
-            : st_launch_batch_track_mix_exact( &s, until_turn, k, d_status, grid(), pairs, stream );
+            ? st_launch_batch_track_mix_fma( &s, until_turn, k, d_status, grid_wgs, pairs, stream )
+            : st_launch_batch_track_mix_exact( &s, until_turn, k, d_status, grid_wgs, pairs, stream );
         return hip_ok( e, "launch batch track (mixed)" );
     }
 
@@ -640,7 +666,9 @@ struct BatchRunner
         {
             if( !select_split_sync( until_turn, stream, n_active, min_turn, tiles ) ) return false;
             if( n_active == 0 ) return true;
-            int64_t pairs = 0;                 // mixed launch: pair waves (0: none)
+            bool mixed = false;                // a mixed launch (mix_shape)
+            int64_t pairs = 0;
+            int mix_grid = 0;
             if( decide )
             {
                 // ppl=auto: two particles per lane run the record walk once for
@@ -648,27 +676,27 @@ struct BatchRunner
                 // ring with 75 % or more of the resident waves busy), but hold
                 // half the waves of one per lane (at 65 % they are 4 % slower);
                 // they are chosen when the particles fill at least 70 % of the
-                // resident waves with 128-particle tiles.  Between one and two
-                // particles per resident lane a mixed launch (mix_pairs) keeps
-                // every wave busy instead; it is decided afresh before each
-                // mixed launch, as particles are lost.
+                // resident waves with 128-particle tiles.  A mixed launch
+                // (mix_shape) spreads the particles evenly over the waves of
+                // every SIMD instead where it applies; it is decided afresh
+                // before each mixed launch, as particles are lost.
                 if( cfg.ppl_auto && !cfg.heavy && !cfg.lds() )
                 {
                     cfg.ppl = ( 100 * n_active >= 70 * 128 * W ) ? 2 : 1;
                     tiles = split_tiles();
-                    pairs = mix_pairs( n_active );
+                    mixed = mix_shape( n_active, pairs, mix_grid );
                 }
                 int64_t const r = until_turn - min_turn;
-                k = pairs > 0 ? equal_turns( r, n_active ) : balanced_turns( tiles, r, n_active );
+                k = mixed ? equal_turns( r, n_active ) : balanced_turns( tiles, r, n_active );
                 if( group == 0 ) limit = 16 + 4 * ( ( tiles + W - 1 ) / W ) * ( ( r + k - 1 ) / k );
-                decide = pairs > 0;
+                decide = mixed;
             }
-            if( pairs > 0 )
+            if( mixed )
             {
                 // one mixed launch per group: every active particle advances k
                 // turns, and the next group decides again
                 if( launches++ >= limit ) return fail( "balanced tracking made no progress" );
-                if( !enqueue_track_mix( until_turn, k, d_status, pairs, stream ) ) return false;
+                if( !enqueue_track_mix( until_turn, k, d_status, pairs, mix_grid, stream ) ) return false;
                 continue;
             }
             int64_t const v = ( until_turn - min_turn + k - 1 ) / k;     // visits per particle
