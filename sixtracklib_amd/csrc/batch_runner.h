@@ -571,8 +571,11 @@ struct BatchRunner
     // 256 consecutive workgroups of 256 on every SIMD, so a grid of w * n_cu
     // workgroups holds w waves on every SIMD, the pair waves m of them.
     //   * More than one particle per resident lane (n > 64 W, W = 32 waves
-    //     per CU): the full grid (w = 8), m the fewest pair waves per SIMD
-    //     that hold n, when n fills at least 90 % of the lanes (part-filled
+    //     per CU): j = ceil(n / (64 x 4 x n_cu)) lanes of 64 per SIMD hold n
+    //     (a pair wave counts two), and of the shapes with w + m = j the one
+    //     with the most pairs and at least one single wave, w = ceil((j +
+    //     1) / 2), is the fastest (1-2 % above 8 waves with the fewest
+    //     pairs); used when n fills at least 90 % of the lanes (part-filled
     //     single waves cost as much as full ones).
     //   * At most one per resident lane: j = ceil(n / (64 x 4 x n_cu))
     //     waves per SIMD hold n one per lane, and one wave per 64-particle
@@ -583,8 +586,8 @@ struct BatchRunner
     //     faster than one per lane when the tiles leave 9 % of the lanes
     //     empty, equal when they fill them).
     // false when the launch does not apply (several seeds, particles at
-    // several turns, a grid other than 8 workgroups of 256 per CU, fewer
-    // than 6 waves per SIMD).
+    // several turns, a grid other than 8 workgroups of 256 per CU, at most
+    // one particle per lane on fewer than 6 waves per SIMD).
     bool mix_shape( int64_t n, int64_t& pairs, int& grid_wgs ) const
     {
         pairs = 0;
@@ -595,17 +598,15 @@ struct BatchRunner
         int64_t const W = waves();
         if( W != 32 * ( int64_t )n_cu ) return false;
         int64_t const per = 64 * 4 * ( int64_t )n_cu;          // one wave per SIMD, one per lane
+        int64_t const j = ( n + per - 1 ) / per;
         if( n > 64 * W )
         {
-            int64_t const m = ( n - 64 * W + per - 1 ) / per;
-            if( m >= 8 ) return false;
-            int64_t const cap = 64 * W + m * per;
-            if( 100 * n < 90 * cap ) return false;
+            int64_t const w = ( j + 2 ) / 2, m = j - w;
+            if( w > 8 || 100 * n < 90 * j * per ) return false;
             pairs = m * 4 * ( int64_t )n_cu;
-            grid_wgs = grid();
+            grid_wgs = ( int )( w * n_cu );
             return true;
         }
-        int64_t const j = ( n + per - 1 ) / per;
         if( j < 6 ) return false;
         int64_t const m = j == 8 ? 2 : 0;
         pairs = m * 4 * ( int64_t )n_cu;
