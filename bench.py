@@ -46,10 +46,10 @@ HBM_PEAK_GBS = 8000.0
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
 # a track launch of the plain 1000-turn job of the 1M-particle ring (mean of
 # its 25 launches, 3 of two particles per lane and 22 mixed:
-# 4,473,222 / 4,828,122 KiB) and one turn of element-by-element
+# 5,015,995 / 5,796,285 KiB) and one turn of element-by-element
 # tracking of the collective ring with 100k particles in the exact build
 # (one 10-turn call / 10: 1,473,305 / 2,052,356,320 KiB)
-TRACK_LAUNCH_HBM_BYTES = 4473222 * 2048 + 4828122 * 1024
+TRACK_LAUNCH_HBM_BYTES = 5015995 * 2048 + 5796285 * 1024
 EBE_TURN_HBM_BYTES = (1473305 * 2048 + 2052356320 * 1024) // 10
 
 
@@ -515,7 +515,7 @@ def main():
                                        "(FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x); waves with two "
                                        "particles per lane read the particle constants and delta and "
                                        "write delta and psigma at each Cavity (registers hold only the "
-                                       "Drift/Multipole state): 13-22 GB per launch at 44-66 GB/s",
+                                       "Drift/Multipole state): 15-22 GB per launch at 52-66 GB/s",
                      "algorithmic_hbm_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "batched": batched,
