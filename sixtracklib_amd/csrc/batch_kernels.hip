@@ -394,8 +394,15 @@ KN( st_k_batch_track_mix )( st_batch_sched_t s, int64_t until_turn, int64_t max_
     int64_t const pairs = pair_waves < W ? pair_waves : W;
     if( w < pairs )
     {
-        int64_t const begin = 128 * w;
-        int64_t const end = begin + 128 < n ? begin + 128 : n;
+        // 128 particles per pair wave; with every wave a pair wave, n is
+        // spread evenly over them instead (at most 128 each)
+        int64_t begin = 128 * w, end = begin + 128 < n ? begin + 128 : n;
+        if( pairs == W )
+        {
+            int64_t const q = n / W, r = n % W;
+            begin = w * q + ( w < r ? w : r );
+            end = begin + q + ( w < r ? 1 : 0 );
+        }
         track_lean_lane2( PDesc{ sd }, list, end, begin, lat, n_elem, until_turn, max_turns, status, wp );
         return;
     }

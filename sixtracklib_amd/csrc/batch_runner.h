@@ -576,7 +576,8 @@ struct BatchRunner
     //     with the most pairs and at least one single wave, w = ceil((j +
     //     1) / 2), is the fastest (1-2 % above 8 waves with the fewest
     //     pairs); used when n fills at least 90 % of the lanes (part-filled
-    //     single waves cost as much as full ones).
+    //     single waves cost as much as full ones).  For j = 16 every wave
+    //     is a pair wave and n is spread evenly over them.
     //   * At most one per resident lane: j = ceil(n / (64 x 4 x n_cu))
     //     waves per SIMD hold n one per lane, and one wave per 64-particle
     //     tile would leave some SIMDs with j waves and the others with
@@ -599,6 +600,15 @@ struct BatchRunner
         if( W != 32 * ( int64_t )n_cu ) return false;
         int64_t const per = 64 * 4 * ( int64_t )n_cu;          // one wave per SIMD, one per lane
         int64_t const j = ( n + per - 1 ) / per;
+        if( n > 64 * W && n <= 128 * W && j == 16 )
+        {
+            // more than 15 pair waves' worth per SIMD: every wave a pair
+            // wave, n spread evenly over them (128-particle tiles would
+            // leave whole waves idle)
+            pairs = W;
+            grid_wgs = grid();
+            return true;
+        }
         if( n > 64 * W )
         {
             int64_t const w = ( j + 2 ) / 2, m = j - w;
