@@ -45,11 +45,11 @@ HBM_PEAK_GBS = 8000.0
 # (KiB per dispatch; a 1 GiB device-to-device copy reads 524,308 KiB of
 # FETCH_SIZE and writes 1,048,576 KiB of WRITE_SIZE, hence x2048 and x1024 B):
 # a track launch of the plain 1000-turn job of the 1M-particle ring (mean of
-# its 25 launches, 3 of two particles per lane and 22 mixed:
-# 5,015,995 / 5,796,285 KiB) and one turn of element-by-element
+# its 25 launches, all of them mixed launches:
+# 5,670,873 / 5,064,299 KiB) and one turn of element-by-element
 # tracking of the collective ring with 100k particles in the exact build
 # (one 10-turn call / 10: 1,473,305 / 2,052,356,320 KiB)
-TRACK_LAUNCH_HBM_BYTES = 5015995 * 2048 + 5796285 * 1024
+TRACK_LAUNCH_HBM_BYTES = 5670873 * 2048 + 5064299 * 1024
 EBE_TURN_HBM_BYTES = (1473305 * 2048 + 2052356320 * 1024) // 10
 
 
@@ -509,13 +509,12 @@ def main():
                      "frac": achieved_tflops / FP64_PEAK_TFLOPS,
                      "traffic": TRACK_LAUNCH_HBM_BYTES,
                      "traffic_source": "rocprofv3 FETCH_SIZE + WRITE_SIZE of the track launches "
-                                       "(st_k_batch_track2_fma, st_k_batch_track_mix_fma) on this "
-                                       "workload, mean of the 25 "
+                                       "(st_k_batch_track_mix_fma) on this workload, mean of the 25 "
                                        "launches of the 1000-turn job, scaled by a 1 GiB device copy "
                                        "(FETCH_SIZE reads 0.5x, WRITE_SIZE 1.0x); waves with two "
                                        "particles per lane read the particle constants and delta and "
                                        "write delta and psigma at each Cavity (registers hold only the "
-                                       "Drift/Multipole state): 15-22 GB per launch at 52-66 GB/s",
+                                       "Drift/Multipole state): 16.8 GB per launch at 58 GB/s",
                      "algorithmic_hbm_bytes_per_launch": alg_bytes},
         "cpu_baseline": cpu,
         "batched": batched,
