@@ -41,8 +41,9 @@
  * per lane and the others with one (the most pair waves that leave one
  * single wave per SIMD); with 6 to 8 waves per SIMD of one
  * particle per lane, the particles spread evenly over 6 or 7 waves of every
- * SIMD (2 of them pairs for 8); results identical), "mix=0|1" (default
- * 1: the mixed launches of ppl=auto), "balance=0|1" (default 1: with compact=1
+ * SIMD (2 of them pairs for 8); the mixed launches in both builds; results
+ * identical), "mix=0|1" (default 1: the mixed launches of ppl=auto),
+ * "balance=0|1" (default 1: with compact=1
  * every launch gives each resident wave one 64-particle tile, the
  * particles furthest behind first, and the host reads the counts back
  * between launches; 0: every chunk hands out all its tiles; results

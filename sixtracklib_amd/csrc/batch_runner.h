@@ -63,8 +63,8 @@ struct Config
     bool ppl_auto = false;       // balanced form: per request, 2 when the particles
                                  // fill at least 70 % of the resident waves with
                                  // 128-particle tiles, else 1 (see run_balanced)
-    bool mix = true;             // with ppl_auto: mixed launches of one and two particles
-                                 // per lane between 64 and 70 % x 128 per resident wave
+    bool mix = false;            // balanced form: mixed launches (mix_shape) where they
+                                 // apply; on with ppl=auto, in either build
     bool heavy = false;          // lattices hold maps outside the lean set
                                  // (set by the owner, one particle per lane)
     bool balance = false;        // run_balanced() available (split lists)
@@ -111,10 +111,11 @@ struct Config
         else if( k == "balance" ) balance = iv != 0;
         else if( k == "ppl" )
         {
-            if( v == "auto" ) { ppl_auto = true; ppl = 1; return true; }
+            if( v == "auto" ) { ppl_auto = true; mix = true; ppl = 1; return true; }
             if( iv != 1 && iv != 2 ) return fail( "ppl (particles per lane) must be 1, 2 or auto" );
             ppl = ( int )iv;
             ppl_auto = false;
+            mix = false;
         }
         else if( k == "mix" ) mix = iv != 0;
         else if( k == "staging" )
@@ -593,7 +594,7 @@ struct BatchRunner
     {
         pairs = 0;
         grid_wgs = 0;
-        if( !cfg.mix || !cfg.ppl_auto || cfg.heavy || cfg.lds() || n_seeds() != 1 ||
+        if( !cfg.mix || cfg.heavy || cfg.lds() || n_seeds() != 1 ||
             h_split == nullptr || h_split[ 1 ] != 0 || cfg.block != 256 || cfg.waves_per_cu != 32 ||
             n_cu <= 0 || n <= 0 ) return false;
         int64_t const W = waves();
@@ -691,9 +692,9 @@ struct BatchRunner
                 // (mix_shape) spreads the particles evenly over the waves of
                 // every SIMD instead where it applies; it is decided afresh
                 // before each mixed launch, as particles are lost.
-                if( cfg.ppl_auto && !cfg.heavy && !cfg.lds() )
+                if( ( cfg.ppl_auto || cfg.mix ) && !cfg.heavy && !cfg.lds() )
                 {
-                    cfg.ppl = ( 100 * n_active >= 70 * 128 * W ) ? 2 : 1;
+                    if( cfg.ppl_auto ) cfg.ppl = ( 100 * n_active >= 70 * 128 * W ) ? 2 : 1;
                     tiles = split_tiles();
                     mixed = mix_shape( n_active, pairs, mix_grid );
                 }

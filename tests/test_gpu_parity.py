@@ -293,13 +293,12 @@ def test_ppl_auto_matches_one_particle_per_lane(fma):
         assert_particles_equal(res["auto"], res["one"], what="auto vs ppl=1, %d particles" % (64 * n_amp))
 
 
-@pytest.mark.parametrize("n_amp,max_sigma,later", [(9375, 20.0, False), (9375, 200.0, False),
-                                                   (9375, 20.0, True), (12500, 20.0, False),
-                                                   (16000, 20.0, False),
-                                                   (7500, 20.0, False), (6250, 20.0, False),
-                                                   (5700, 20.0, False)])
-def test_mixed_launches_match_one_particle_per_lane(n_amp, max_sigma, later):
-    """With ppl=auto in the FMA build, a request of one particle set whose
+@pytest.mark.parametrize("n_amp,max_sigma,later,fma", [
+    (9375, 20.0, False, 1), (9375, 200.0, False, 1), (9375, 20.0, True, 1), (12500, 20.0, False, 1),
+    (16000, 20.0, False, 1), (7500, 20.0, False, 1), (6250, 20.0, False, 1), (5700, 20.0, False, 1),
+    (9375, 200.0, False, 0), (16000, 20.0, False, 0), (7500, 20.0, False, 0)])
+def test_mixed_launches_match_one_particle_per_lane(n_amp, max_sigma, later, fma):
+    """With ppl=auto (either build), a request of one particle set whose
     active particles sit at one turn runs mixed launches: some waves of
     every SIMD take two particles per lane, the others share the rest
     evenly, one per lane.  With 8,192 resident waves: from 524,289 to
@@ -313,8 +312,9 @@ def test_mixed_launches_match_one_particle_per_lane(n_amp, max_sigma, later):
     the next ones mixed); 800,000 particles (6 pairs of 7 waves), 1,024,000
     (every wave a pair wave, the particles spread evenly), and
     480,000, 400,000 and 364,800 particles (6 waves with 2 pairs, 7 waves,
-    6 waves per SIMD) to 20 sigma.  Results equal the one-particle
-    kernel's bit for bit."""
+    6 waves per SIMD) to 20 sigma; in the exact build the 200-sigma,
+    1,024,000 and 480,000 cases.  Results equal the one-particle kernel's
+    bit for bit."""
     from sixtracklib_amd.trackjob import TrackJob
     from sixtracklib_amd.particles import particles_in
     eb = lattices.lhc_like_ring(seed=3, n_cells=8, tune=8.31 * 8 / 40)
@@ -327,7 +327,7 @@ def test_mixed_launches_match_one_particle_per_lane(n_amp, max_sigma, later):
         at[::3] = 2
         p.at_turn[:] = at
     res = {}
-    for key, cfg in (("mixed", "fma=1"), ("one", "fma=1 ppl=1")):
+    for key, cfg in (("mixed", "fma=%d" % fma), ("one", "fma=%d ppl=1" % fma)):
         buf = _copy(pb)
         job = TrackJob(eb, buf, config_str=cfg)
         for t in (3, 7, 12):
