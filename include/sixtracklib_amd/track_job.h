@@ -39,9 +39,9 @@
  * tiles and one otherwise, and between one and two particles per resident
  * lane it runs mixed launches, some waves of every SIMD with two particles
  * per lane and the others with one (the most pair waves that leave one
- * single wave per SIMD); with 6 to 8 waves per SIMD of one
- * particle per lane, the particles spread evenly over 6 or 7 waves of every
- * SIMD (2 of them pairs for 8); the mixed launches in both builds; results
+ * single wave per SIMD); with 2 to 8 waves per SIMD of one particle per
+ * lane, the particles spread evenly over that many waves of every SIMD (6
+ * of them, 2 pairs, for 8); the mixed launches in both builds; results
  * identical), "mix=0|1" (default 1: the mixed launches of ppl=auto),
  * "balance=0|1" (default 1: with compact=1
  * every launch gives each resident wave one 64-particle tile, the

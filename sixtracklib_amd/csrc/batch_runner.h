@@ -582,14 +582,15 @@ struct BatchRunner
     //   * At most one per resident lane: j = ceil(n / (64 x 4 x n_cu))
     //     waves per SIMD hold n one per lane, and one wave per 64-particle
     //     tile would leave some SIMDs with j waves and the others with
-    //     j - 1, idle once their waves end.  For j = 6 or 7 the launch
+    //     j - 1, idle once their waves end.  For 2 <= j <= 7 the launch
     //     spreads n evenly over j waves on every SIMD (m = 0); for j = 8,
-    //     over 6 waves of which m = 2 pairs (the same lanes, 4 % to 5 %
-    //     faster than one per lane when the tiles leave 9 % of the lanes
-    //     empty, equal when they fill them).
+    //     over 6 waves of which m = 2 pairs (the same lanes; up to 5 %
+    //     faster than one wave per tile in the FMA build and 8 % in the
+    //     exact build where the tiles leave lanes empty, equal where they
+    //     fill them).
     // false when the launch does not apply (several seeds, particles at
     // several turns, a grid other than 8 workgroups of 256 per CU, at most
-    // one particle per lane on fewer than 6 waves per SIMD).
+    // one wave per SIMD).
     bool mix_shape( int64_t n, int64_t& pairs, int& grid_wgs ) const
     {
         pairs = 0;
@@ -618,7 +619,7 @@ struct BatchRunner
             grid_wgs = ( int )( w * n_cu );
             return true;
         }
-        if( j < 6 ) return false;
+        if( j < 2 ) return false;
         int64_t const m = j == 8 ? 2 : 0;
         pairs = m * 4 * ( int64_t )n_cu;
         grid_wgs = ( int )( ( j - m ) * n_cu );

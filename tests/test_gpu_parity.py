@@ -296,6 +296,7 @@ def test_ppl_auto_matches_one_particle_per_lane(fma):
 @pytest.mark.parametrize("n_amp,max_sigma,later,fma", [
     (9375, 20.0, False, 1), (9375, 200.0, False, 1), (9375, 20.0, True, 1), (12500, 20.0, False, 1),
     (16000, 20.0, False, 1), (7500, 20.0, False, 1), (6250, 20.0, False, 1), (5700, 20.0, False, 1),
+    (2350, 20.0, False, 1),
     (9375, 200.0, False, 0), (16000, 20.0, False, 0), (7500, 20.0, False, 0)])
 def test_mixed_launches_match_one_particle_per_lane(n_amp, max_sigma, later, fma):
     """With ppl=auto (either build), a request of one particle set whose
@@ -303,18 +304,18 @@ def test_mixed_launches_match_one_particle_per_lane(n_amp, max_sigma, later, fma
     every SIMD take two particles per lane, the others share the rest
     evenly, one per lane.  With 8,192 resident waves: from 524,289 to
     983,040 particles (9 to 15 lanes of 64 per SIMD) m pairs of w waves per
-    SIMD, w = ceil((j + 1) / 2); up to 1,048,576 all 8 waves pairs; from 327,681 to 524,288 (6 to 8 waves per
-    SIMD one per lane) 6 or 7 waves per SIMD, two of them pairs for 8.  On
-    a 400-element ring: 600,000 particles (4 pairs of 6 waves) to 20 sigma
-    (all requests mixed), to 200 sigma (most lost in the first turns, the
-    later requests on the smaller grids or one per lane), and with every
-    third particle starting two turns later (the first request not mixed,
-    the next ones mixed); 800,000 particles (6 pairs of 7 waves), 1,024,000
-    (every wave a pair wave, the particles spread evenly), and
-    480,000, 400,000 and 364,800 particles (6 waves with 2 pairs, 7 waves,
-    6 waves per SIMD) to 20 sigma; in the exact build the 200-sigma,
-    1,024,000 and 480,000 cases.  Results equal the one-particle kernel's
-    bit for bit."""
+    SIMD, w = ceil((j + 1) / 2); up to 1,048,576 all 8 waves pairs; from
+    65,537 to 524,288 (2 to 8 waves per SIMD one per lane) j waves per
+    SIMD, or 6 with two pairs for 8.  On a 400-element ring: 600,000
+    particles (4 pairs of 6 waves) to 20 sigma (all requests mixed), to 200
+    sigma (most lost in the first turns, the later requests on smaller
+    grids), and with every third particle starting two turns later (the
+    first request not mixed, the next ones mixed); 800,000 particles (6
+    pairs of 7 waves), 1,024,000 (every wave a pair wave, the particles
+    spread evenly), and 480,000, 400,000, 364,800 and 150,400 particles (6
+    waves with 2 pairs, 7, 6 and 3 waves per SIMD) to 20 sigma; in the
+    exact build the 200-sigma, 1,024,000 and 480,000 cases.  Results equal
+    the one-particle kernel's bit for bit."""
     from sixtracklib_amd.trackjob import TrackJob
     from sixtracklib_amd.particles import particles_in
     eb = lattices.lhc_like_ring(seed=3, n_cells=8, tune=8.31 * 8 / 40)
