@@ -341,6 +341,28 @@ def test_mixed_launches_match_one_particle_per_lane(n_amp, max_sigma, later, fma
     assert_particles_equal(res["mixed"], res["one"], what="mixed vs ppl=1")
 
 
+@pytest.mark.parametrize("fma", [0, 1])
+def test_mix_off_matches_mixed_launches(fma):
+    """mix=0 turns the mixed launches off (ppl=auto then launches 64- or
+    128-particle tiles, one wave each): 480,000 particles on a 400-element
+    ring to 200 sigma, where the default takes mixed launches of every kind
+    as particles are lost.  Results are identical bit for bit."""
+    from sixtracklib_amd.trackjob import TrackJob
+    eb = lattices.lhc_like_ring(seed=3, n_cells=8, tune=8.31 * 8 / 40)
+    sig = lattices.twiss_sigma(beta=236.0)
+    pb, _ = lattices.polar_grid(7500, 64, sigma_x=sig, sigma_y=sig, max_sigma=200.0, delta=2.7e-4)
+    res = {}
+    for key, cfg in (("mixed", "fma=%d" % fma), ("off", "fma=%d mix=0" % fma)):
+        buf = _copy(pb)
+        job = TrackJob(eb, buf, config_str=cfg)
+        for t in (3, 7, 12):
+            assert job.track_until(t).last_track_status == 0
+        job.collect()
+        res[key] = _pdict(buf)
+    assert np.count_nonzero(res["off"]["state"] == 0) > 50000
+    assert_particles_equal(res["mixed"], res["off"], what="mixed vs mix=0")
+
+
 @pytest.mark.parametrize("seed", [0, 1])
 def test_kitchen_sink_exact_bitwise(seed):
     eb = lattices.kitchen_sink(seed)
