@@ -48,9 +48,9 @@ HBM_PEAK_GBS = 8000.0
 # its 25 launches, all of them mixed launches:
 # 5,670,873 / 5,064,299 KiB) and one turn of element-by-element
 # tracking of the collective ring with 100k particles in the exact build
-# (one 10-turn call / 10: 1,473,305 / 2,052,356,320 KiB)
+# (one 10-turn call / 10: 1,473,333 / 2,052,266,253 KiB)
 TRACK_LAUNCH_HBM_BYTES = 5670873 * 2048 + 5064299 * 1024
-EBE_TURN_HBM_BYTES = (1473305 * 2048 + 2052356320 * 1024) // 10
+EBE_TURN_HBM_BYTES = (1473333 * 2048 + 2052266253 * 1024) // 10
 
 
 def sigma_at_start(eb):

@@ -259,6 +259,10 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
     {
         cptr_t r = a.lat;
         uint64_t h = r[ 0 ];
+        // element by element (one wave per SIMD, nothing to hide a scalar
+        // load's latency behind): a record's first parameter word, a
+        // drift's length, is read with its header, one element ahead
+        uint64_t p0 = MODE == MODE_EBE ? r[ ST_REC_PAYLOAD ] : 0u;
         int e_end = n_elem;
         bool const turn_ok = pid_ok && a.cfg.min_turn <= at_turn &&
                              ( a.cfg.max_turn >= at_turn || a.cfg.is_rolling );
@@ -267,6 +271,7 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
         {
             cptr_t const nr = r + ST_REC_LEN( h );
             uint64_t const hn = nr[ 0 ];          // next header (stream is padded)
+            uint64_t const pn = MODE == MODE_EBE ? nr[ ST_REC_PAYLOAD ] : 0u;
             unsigned const op = ST_REC_OP( h );
             cptr_t const prm = r + ST_REC_PAYLOAD;
             if( MODE == MODE_EBE )
@@ -295,7 +300,7 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
             uint64_t bad = 0u, failm = 0u;        // lost here / map failed
             if( op == ST_OP_DRIFT )
             {
-                drift( L, rd( prm, 0 ) );
+                drift( L, MODE == MODE_EBE ? __longlong_as_double( ( long long )p0 ) : rd( prm, 0 ) );
                 bad = ballot( !limit_global_ok( L ) );
             }
             else if( op == ST_OP_MULTIPOLE )
@@ -325,6 +330,7 @@ ST_DEV void track_body( const st_dev_particles_t& P, int64_t i, const Args& a )
             }
             r = nr;
             h = hn;
+            p0 = pn;
         }
         if( MODE == MODE_LINE ) break;
         if( lane_in( act ) ) { ++at_turn; L.s = 0.0; }
