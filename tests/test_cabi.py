@@ -418,3 +418,16 @@ def test_cuda_trackjob_without_device_raises():
     pb, _ = lattices.gaussian_beam(10, seed=0)
     with pytest.raises(RuntimeError):
         CudaTrackJob(eb, pb)
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """bench.py measures one GPU: --gpus above 1 exits with status 2 and a
+    message, before any device work."""
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    p = subprocess.run([sys.executable, os.path.join(root, "bench.py"), "--gpus", "2"],
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=300)
+    assert p.returncode == 2
+    assert p.stdout == ""
+    assert "one GPU" in p.stderr
